@@ -1,0 +1,252 @@
+/*
+ * reikna_amd.h — C-ABI of the MI355X (gfx950) FFT library.
+ *
+ * The reference (reikna) has no native code: its FFT is a Python planner that
+ * renders Mako templates and enqueues them through grunnur.  The native
+ * boundary it would bind is the kernel-level call site
+ *
+ *     plan.kernel_call(TEMPLATE.get_def("fft_local"|"fft_global"),
+ *                      [out, in, (kweights), inverse], ...)
+ *        reikna/fft/fft.py:704-711, executed by StaticKernel.__call__
+ *        at reikna/core/computation.py:795
+ *
+ * together with grunnur's device runtime (Array / Queue / Buffer:
+ * reikna/core/computation.py:7-16,401,444,683-693).  Every entry point
+ * below replaces one of those pieces; the comment on each cites it.
+ *
+ * Conventions
+ *   - Every function returns an int status: 0 (RK_OK) on success, a negative
+ *     RK_ERR_* code otherwise; rk_last_error() then returns a thread-local
+ *     message.  Nothing here throws across the boundary.
+ *   - Strides and offsets are in ELEMENTS (complex64 = 8 bytes), not bytes.
+ *   - Streams are hipStream_t passed as void* (NULL = the default stream).
+ *   - Plans are immutable after creation: concurrent rk_fft_execute calls on
+ *     different streams with different workspaces are safe.
+ */
+#ifndef REIKNA_AMD_H
+#define REIKNA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RK_MAX_DIMS 8
+
+enum rk_status {
+    RK_OK = 0,
+    RK_ERR_INVALID = -1,     /* bad argument (Python maps it to ValueError) */
+    RK_ERR_HIP = -2,         /* a HIP runtime call failed */
+    RK_ERR_NO_KERNEL = -3,   /* no compiled kernel for a pass and JIT failed */
+    RK_ERR_UNSUPPORTED = -4, /* valid request this build does not implement */
+    RK_ERR_BUDGET = -5       /* request exceeds an HBM budget */
+};
+
+enum rk_dtype { RK_C64 = 0 };
+
+/* ------------------------------------------------------------------ */
+/* Errors                                                              */
+/* ------------------------------------------------------------------ */
+
+/* Thread-local text of the last failure on the calling thread. */
+const char* rk_last_error(void);
+
+/* Library version string, e.g. "reikna_amd 0.1 gfx950". */
+const char* rk_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Device runtime — replaces grunnur Array/Queue/Buffer                */
+/* (reikna/core/computation.py:401 Array.from_host, :689 Array.empty_like, */
+/*  :594-597 Queue.synchronize, :686 Buffer.allocate)                  */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    char name[256];
+    char arch[64];              /* e.g. "gfx950" */
+    int compute_units;
+    int wavefront_size;
+    int max_threads_per_block;
+    size_t lds_per_block;       /* bytes of LDS a work-group may use */
+    size_t total_global_mem;
+    int l2_cache_bytes;
+    int clock_khz;
+    int memory_clock_khz;
+    int memory_bus_width;
+} rk_device_info;
+
+int rk_device_count(int* count);
+int rk_set_device(int device);
+int rk_get_device_info(int device, rk_device_info* info);
+
+int rk_malloc(void** ptr, size_t bytes);
+int rk_free(void* ptr);
+int rk_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream);
+int rk_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream);
+int rk_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream);
+int rk_memset(void* dst, int value, size_t bytes, void* stream);
+int rk_mem_info(size_t* free_bytes, size_t* total_bytes);
+
+int rk_stream_create(void** stream);
+int rk_stream_destroy(void* stream);
+int rk_stream_synchronize(void* stream);
+int rk_device_synchronize(void);
+
+int rk_event_create(void** event);
+int rk_event_destroy(void* event);
+int rk_event_record(void* event, void* stream);
+int rk_event_synchronize(void* event);
+int rk_event_query(void* event, int* done);
+int rk_event_elapsed_ms(void* start, void* stop, float* ms);
+int rk_stream_wait_event(void* stream, void* event);
+
+/* ------------------------------------------------------------------ */
+/* Run-time compiled modules — replaces grunnur StaticKernel's driver JIT */
+/* (reikna/core/computation.py:579-591)                                */
+/* ------------------------------------------------------------------ */
+
+/* Compile HIP C++ source for the current device with hiprtc.  The generated
+   code object is cached on disk (RK_CACHE_DIR, default ~/.cache/reikna_amd),
+   keyed by source, options and target.  include_dir is added as -I. */
+int rk_module_compile(const char* source, const char* options,
+                      void** module, char* log, size_t log_size);
+int rk_module_get_function(void* module, const char* name, void** function);
+int rk_module_unload(void* module);
+/* Launch a run-time compiled kernel: args is a packed argument buffer laid out
+   exactly as the kernel's parameter list (HIP_LAUNCH_PARAM_BUFFER_POINTER). */
+int rk_launch(void* function, const uint32_t grid[3], const uint32_t block[3],
+              uint32_t shared_bytes, void* stream, const void* args,
+              size_t args_bytes);
+
+/* ------------------------------------------------------------------ */
+/* FFT plans — replace FFT._build_plan + the fft_local/fft_global chain */
+/* (reikna/fft/fft.py:532-584 get_fft_kernels, :664-721 buffer routing, */
+/*  :723-756 _build_plan, fft.mako:750-814 fft_local, :816-1038 fft_global) */
+/* ------------------------------------------------------------------ */
+
+typedef struct rk_fft_plan rk_fft_plan;
+
+typedef struct {
+    int dtype;                          /* RK_C64 */
+    int ndim;                           /* 1..RK_MAX_DIMS */
+    int64_t shape[RK_MAX_DIMS];
+    int64_t in_strides[RK_MAX_DIMS];    /* elements */
+    int64_t out_strides[RK_MAX_DIMS];   /* elements */
+    int64_t in_offset;                  /* elements */
+    int64_t out_offset;                 /* elements */
+    int naxes;                          /* 0 = transform over all axes */
+    int axes[RK_MAX_DIMS];              /* non-negative; order as given */
+    uint32_t flags;                     /* RK_FFT_* below */
+} rk_fft_desc;
+
+/* flags */
+#define RK_FFT_FORCE_BLUESTEIN 0x1u     /* use chirp-z even for smooth lengths */
+#define RK_FFT_FORCE_MULTIPASS 0x2u     /* split every axis into >= 2 passes */
+#define RK_FFT_NO_JIT          0x4u     /* never run hiprtc for missing lengths */
+#define RK_FFT_NORM_FORWARD    0x8u     /* scale the forward result by 1/prod(N), not the inverse */
+#define RK_FFT_OUT_LAST_ONLY   0x10u    /* only the last pass touches `out`: earlier axes finish in workspace */
+
+/* Build a plan: radix factorisation, pass chain, buffer routing, and all
+   device tables (twiddles in FP64 rounded once to FP32, Bluestein chirp from
+   exact integer n^2 mod 2N, Bluestein kernel spectrum by an FP64 host FFT).
+   Replaces FFT.__init__ + _build_plan (fft.py:630-645, :723-756) and the
+   kweights persistent array (fft.py:216-236, :691-692). */
+int rk_fft_plan_create(const rk_fft_desc* desc, rk_fft_plan** plan);
+
+/* Bytes of scratch device memory rk_fft_execute needs (0 if none).
+   Replaces the plan's temp_array allocations (computation.py:404-434,
+   fft.py:681-689). */
+int rk_fft_plan_workspace_size(const rk_fft_plan* plan, size_t* bytes);
+
+/* Enqueue the transform on `stream`.  inverse = 0: forward, exp(-2 pi i kn/N);
+   inverse != 0: backward with 1/prod(N) normalisation, so that
+   IFFT(FFT(x)) = x (fft.mako:741-747).  in == out is allowed.
+   `workspace` must hold rk_fft_plan_workspace_size bytes (may be NULL when
+   that is 0).  Asynchronous.  Replaces ComputationCallable.__call__ over the
+   FFT kernel chain (computation.py:766-801). */
+int rk_fft_execute(const rk_fft_plan* plan, void* out, const void* in,
+                   int inverse, void* workspace, void* stream);
+
+/* Same, for plans whose first/last pass kernels were replaced by fused
+   (transformation-carrying) kernels with rk_fft_plan_set_pass_kernel: the
+   byte blobs are copied into the user-data slots of the first pass
+   (load side) and the last pass (store side).  Each is at most
+   RK_FUSED_BLOB_BYTES.  Replaces the load_combined_idx/store_combined_idx
+   splicing of transformations (fft.mako:220-221,871-872,1000-1002). */
+#define RK_FUSED_BLOB_BYTES 512
+int rk_fft_execute_fused(const rk_fft_plan* plan, void* out, const void* in,
+                         int inverse, void* workspace, void* stream,
+                         const void* load_blob, size_t load_bytes,
+                         const void* store_blob, size_t store_bytes);
+
+int rk_fft_plan_destroy(rk_fft_plan* plan);
+
+/* Plan introspection (for tests, benches and the fused-kernel generator). */
+typedef struct {
+    int L;                /* DFT length computed by this pass */
+    int C;                /* transforms per work-group tile */
+    int threads;          /* work-group size */
+    int kind;             /* 0 = Stockham pass, 1 = single-kernel Bluestein, 2 = copy */
+    int load_mode;        /* 0 direct, 1 column-staged, 2 flat-staged */
+    int store_mode;
+    int src;              /* 0 = in, 1 = out, 2 = ws0, 3 = ws1 */
+    int dst;
+    int axis;             /* array axis this pass belongs to */
+    uint32_t pre_ops, post_ops;
+    uint64_t n_inst;      /* transforms (instances) in the pass */
+    uint64_t grid;        /* work-groups launched */
+    uint64_t axis_len;    /* N of the axis (M for Bluestein passes) */
+    uint64_t Ns;          /* product of the lengths of earlier passes of this axis */
+    int lds_bytes;
+    int jit;              /* 1 if the kernel came from the run-time compiler */
+    int variant;          /* compile-time kernel variant (pre-op | post-op << 2 | bluestein << 4) */
+    int64_t in_dims[3];   /* folded (O, axis length read, I) of the pass input */
+} rk_fft_pass_info;
+
+int rk_fft_plan_num_passes(const rk_fft_plan* plan, int* n);
+int rk_fft_plan_pass_info(const rk_fft_plan* plan, int pass, rk_fft_pass_info* info);
+
+/* Replace the kernel of one pass with a run-time compiled one whose
+   parameter list is the same pass-argument struct (fused transformations). */
+int rk_fft_plan_set_pass_kernel(rk_fft_plan* plan, int pass, void* function);
+
+/* Device-side duration of the last rk_fft_execute of `pass` measured with
+   hipEvents when RK_FFT_TIMING was set at plan creation (ms), else -1. */
+int rk_fft_plan_set_timing(rk_fft_plan* plan, int enable);
+int rk_fft_plan_pass_time_ms(const rk_fft_plan* plan, int pass, float* ms);
+
+/* ------------------------------------------------------------------ */
+/* Batched uneven transforms — one persistent launch (no reference      */
+/* counterpart: reikna compiles one computation per shape).             */
+/* ------------------------------------------------------------------ */
+
+typedef struct rk_batch_plan rk_batch_plan;
+
+typedef struct {
+    int64_t length;      /* 1-D transform length N (any N >= 1) */
+    int64_t count;       /* number of transforms of this length */
+    int64_t in_offset;   /* element offset of the first transform in `in` */
+    int64_t out_offset;  /* element offset of the first transform in `out` */
+    int64_t in_dist;     /* elements between consecutive transforms (>= N) */
+    int64_t out_dist;
+    uint32_t flags;      /* RK_BATCH_SCALE: multiply result by 1/N */
+} rk_batch_item;
+
+#define RK_BATCH_SCALE 0x1u
+
+int rk_batch_plan_create(const rk_batch_item* items, int n_items,
+                         rk_batch_plan** plan);
+int rk_batch_plan_workspace_size(const rk_batch_plan* plan, size_t* bytes);
+int rk_batch_execute(const rk_batch_plan* plan, void* out, const void* in,
+                     int inverse, void* workspace, void* stream);
+int rk_batch_plan_destroy(rk_batch_plan* plan);
+/* Number of work items / launches the batch plan uses (introspection). */
+int rk_batch_plan_stats(const rk_batch_plan* plan, int64_t* work_items,
+                        int* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* REIKNA_AMD_H */

@@ -1,0 +1,573 @@
+// fft_device.hpp — device building blocks of the gfx950 FFT.
+//
+// One kernel template, fft_pass_body<L, C, V, IO>, computes a tile of C
+// independent DFTs of compile-time length L inside one work-group (V is the
+// compile-time variant: pre-op, post-op, Bluestein; see Variant below):
+//
+//   * each DFT ("instance") is worked on by T = L / R0 threads, R0 being the
+//     largest radix of the compile-time radix plan of L (16, 8, 4, 2, 7, 5, 3);
+//   * the stages are Stockham autosort steps: a thread reads R values at
+//     j + r*L/R from LDS, multiplies by the stage twiddles W_{Ns*R}^{r*(j%Ns)}
+//     (a table built in FP64 and rounded once), does an in-register DFT_R and
+//     writes them to (j/Ns)*Ns*R + j%Ns + r*Ns, so the result comes out in
+//     natural order with no bit reversal;
+//   * global memory is walked in one of three ways per side (rk_pass_args
+//     load_mode/store_mode): DIRECT (each thread its own elements: coalesced
+//     when the axis is contiguous), COLUMN (lanes across adjacent instances,
+//     staged through LDS: coalesced when instances are adjacent, i.e. strided
+//     axes and the inner passes of a multi-pass transform), FLAT (the tile is
+//     one contiguous block, streamed linearly through LDS);
+//   * the Bluestein variant runs the whole chirp-z convolution in one tile:
+//     chirp on load, forward DFT_L, multiply by the kernel spectrum and
+//     conjugate, forward DFT_L again, conjugate, chirp and truncate on store.
+//
+// The inverse transform is the forward one on conjugated data
+// (IFFT(x) = conj(FFT(conj x)) / N), so the direction is the run-time
+// in_sign / out_sign / out_scale of the pass arguments, not a kernel variant.
+//
+// The IO policy supplies the element load/store so that fused input/output
+// transformations (generated at run time) reuse the same kernel body.
+#ifndef RK_FFT_DEVICE_HPP
+#define RK_FFT_DEVICE_HPP
+
+#if !defined(__HIPCC_RTC__)
+#include <hip/hip_runtime.h>
+#endif
+
+#include "fft_args.h"
+
+#define RK_HD __host__ __device__ __forceinline__
+
+namespace rkfft {
+
+// ------------------------------------------------------------------------
+// complex arithmetic on float2 (x = re, y = im)
+// ------------------------------------------------------------------------
+RK_HD float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+RK_HD float2 c_sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+RK_HD float2 c_mul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+RK_HD float2 c_conj(float2 a) { return make_float2(a.x, -a.y); }
+RK_HD float2 c_mul_mi(float2 a) { return make_float2(a.y, -a.x); }   // a * (-i)
+RK_HD float2 c_scale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+// Butterfly constants: double literals, rounded once to float.
+#define RK_C8 ((float)0.7071067811865476)     // cos(pi/4)
+#define RK_C16 ((float)0.9238795325112867)    // cos(pi/8)
+#define RK_S16 ((float)0.3826834323650898)    // sin(pi/8)
+#define RK_S3 ((float)0.8660254037844387)     // sin(2pi/3)
+#define RK_C5_1 ((float)0.30901699437494745)  // cos(2pi/5)
+#define RK_C5_2 ((float)-0.8090169943749473)  // cos(4pi/5)
+#define RK_S5_1 ((float)0.9510565162951535)   // sin(2pi/5)
+#define RK_S5_2 ((float)0.5877852522924732)   // sin(4pi/5)
+#define RK_C7_1 ((float)0.6234898018587336)
+#define RK_C7_2 ((float)-0.22252093395631434)
+#define RK_C7_3 ((float)-0.900968867902419)
+#define RK_S7_1 ((float)0.7818314824680298)
+#define RK_S7_2 ((float)0.9749279121818236)
+#define RK_S7_3 ((float)0.43388373911755823)
+
+// ------------------------------------------------------------------------
+// in-register forward DFTs, y_k = sum_n v_n exp(-2 pi i n k / R), in place
+// ------------------------------------------------------------------------
+RK_HD void dft2(float2& a, float2& b) {
+    float2 t = a;
+    a = c_add(t, b);
+    b = c_sub(t, b);
+}
+
+RK_HD void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+    float2 t0 = c_add(a0, a2), t1 = c_sub(a0, a2);
+    float2 t2 = c_add(a1, a3), t3 = c_mul_mi(c_sub(a1, a3));
+    a0 = c_add(t0, t2);
+    a2 = c_sub(t0, t2);
+    a1 = c_add(t1, t3);
+    a3 = c_sub(t1, t3);
+}
+
+// multiply by W8^1 = (1 - i)/sqrt2 and W8^3 = (-1 - i)/sqrt2
+RK_HD float2 c_w8_1(float2 a) { return make_float2(RK_C8 * (a.x + a.y), RK_C8 * (a.y - a.x)); }
+RK_HD float2 c_w8_3(float2 a) { return make_float2(RK_C8 * (a.y - a.x), -RK_C8 * (a.x + a.y)); }
+
+RK_HD void dft8(float2* v) {
+    float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4(e0, e1, e2, e3);
+    dft4(o0, o1, o2, o3);
+    o1 = c_w8_1(o1);
+    o2 = c_mul_mi(o2);
+    o3 = c_w8_3(o3);
+    v[0] = c_add(e0, o0); v[4] = c_sub(e0, o0);
+    v[1] = c_add(e1, o1); v[5] = c_sub(e1, o1);
+    v[2] = c_add(e2, o2); v[6] = c_sub(e2, o2);
+    v[3] = c_add(e3, o3); v[7] = c_sub(e3, o3);
+}
+
+// 16 = 4 x 4: DFT4 over n2 for each n1 (x[n1 + 4 n2]), twiddle W16^{n1 k2},
+// DFT4 over n1 for each k2, output y[k2 + 4 k1].
+RK_HD void dft16(float2* v) {
+    dft4(v[0], v[4], v[8], v[12]);
+    dft4(v[1], v[5], v[9], v[13]);
+    dft4(v[2], v[6], v[10], v[14]);
+    dft4(v[3], v[7], v[11], v[15]);
+    const float2 w1 = make_float2(RK_C16, -RK_S16);
+    const float2 w3 = make_float2(RK_S16, -RK_C16);
+    const float2 w9 = make_float2(-RK_C16, RK_S16);
+    v[5] = c_mul(v[5], w1);
+    v[9] = c_w8_1(v[9]);
+    v[13] = c_mul(v[13], w3);
+    v[6] = c_w8_1(v[6]);
+    v[10] = c_mul_mi(v[10]);
+    v[14] = c_w8_3(v[14]);
+    v[7] = c_mul(v[7], w3);
+    v[11] = c_w8_3(v[11]);
+    v[15] = c_mul(v[15], w9);
+    dft4(v[0], v[1], v[2], v[3]);
+    dft4(v[4], v[5], v[6], v[7]);
+    dft4(v[8], v[9], v[10], v[11]);
+    dft4(v[12], v[13], v[14], v[15]);
+    // v[4 k2 + k1] holds y[k2 + 4 k1]: transpose the 4x4 block
+    float2 t;
+    t = v[1]; v[1] = v[4]; v[4] = t;
+    t = v[2]; v[2] = v[8]; v[8] = t;
+    t = v[3]; v[3] = v[12]; v[12] = t;
+    t = v[6]; v[6] = v[9]; v[9] = t;
+    t = v[7]; v[7] = v[13]; v[13] = t;
+    t = v[11]; v[11] = v[14]; v[14] = t;
+}
+
+RK_HD void dft3(float2* v) {
+    float2 t = c_add(v[1], v[2]);
+    float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
+    float2 d = c_sub(v[1], v[2]);
+    float2 s = make_float2(RK_S3 * d.y, -RK_S3 * d.x);   // -i sin(2pi/3) d
+    v[0] = c_add(v[0], t);
+    v[1] = c_add(m, s);
+    v[2] = c_sub(m, s);
+}
+
+RK_HD void dft5(float2* v) {
+    float2 t1 = c_add(v[1], v[4]), t2 = c_add(v[2], v[3]);
+    float2 t3 = c_sub(v[1], v[4]), t4 = c_sub(v[2], v[3]);
+    float2 a0 = v[0];
+    float2 b1 = make_float2(a0.x + RK_C5_1 * t1.x + RK_C5_2 * t2.x, a0.y + RK_C5_1 * t1.y + RK_C5_2 * t2.y);
+    float2 b2 = make_float2(a0.x + RK_C5_2 * t1.x + RK_C5_1 * t2.x, a0.y + RK_C5_2 * t1.y + RK_C5_1 * t2.y);
+    // u1 = s1 t3 + s2 t4, u2 = s2 t3 - s1 t4; y1 = b1 - i u1, y2 = b2 - i u2
+    float2 u1 = make_float2(RK_S5_1 * t3.x + RK_S5_2 * t4.x, RK_S5_1 * t3.y + RK_S5_2 * t4.y);
+    float2 u2 = make_float2(RK_S5_2 * t3.x - RK_S5_1 * t4.x, RK_S5_2 * t3.y - RK_S5_1 * t4.y);
+    v[0] = make_float2(a0.x + t1.x + t2.x, a0.y + t1.y + t2.y);
+    v[1] = c_add(b1, c_mul_mi(u1));
+    v[4] = c_sub(b1, c_mul_mi(u1));
+    v[2] = c_add(b2, c_mul_mi(u2));
+    v[3] = c_sub(b2, c_mul_mi(u2));
+}
+
+// Odd prime radix 7: y_k = A_k - i B_k, y_{7-k} = A_k + i B_k with
+// A_k = a0 + sum_n cos(2 pi n k/7)(a_n + a_{7-n}), B_k = sum_n sin(2 pi n k/7)(a_n - a_{7-n}).
+RK_HD void dft7(float2* v) {
+    const float c[4] = {1.0f, RK_C7_1, RK_C7_2, RK_C7_3};
+    const float s[4] = {0.0f, RK_S7_1, RK_S7_2, RK_S7_3};
+    float2 p[4], m[4];
+    for (int n = 1; n <= 3; ++n) {
+        p[n] = c_add(v[n], v[7 - n]);
+        m[n] = c_sub(v[n], v[7 - n]);
+    }
+    float2 a0 = v[0];
+    v[0] = make_float2(a0.x + p[1].x + p[2].x + p[3].x, a0.y + p[1].y + p[2].y + p[3].y);
+    for (int k = 1; k <= 3; ++k) {
+        float2 A = a0, B = make_float2(0.0f, 0.0f);
+        for (int n = 1; n <= 3; ++n) {
+            int nk = (n * k) % 7;
+            float cc = nk <= 3 ? c[nk] : c[7 - nk];
+            float ss = nk <= 3 ? s[nk] : -s[7 - nk];
+            A.x += cc * p[n].x; A.y += cc * p[n].y;
+            B.x += ss * m[n].x; B.y += ss * m[n].y;
+        }
+        v[k] = c_add(A, c_mul_mi(B));
+        v[7 - k] = c_sub(A, c_mul_mi(B));
+    }
+}
+
+template <int R> RK_HD void dft(float2* v);
+template <> RK_HD void dft<1>(float2*) {}
+template <> RK_HD void dft<2>(float2* v) { dft2(v[0], v[1]); }
+template <> RK_HD void dft<3>(float2* v) { dft3(v); }
+template <> RK_HD void dft<4>(float2* v) { dft4(v[0], v[1], v[2], v[3]); }
+template <> RK_HD void dft<5>(float2* v) { dft5(v); }
+template <> RK_HD void dft<7>(float2* v) { dft7(v); }
+template <> RK_HD void dft<8>(float2* v) { dft8(v); }
+template <> RK_HD void dft<16>(float2* v) { dft16(v); }
+
+// ------------------------------------------------------------------------
+// compile-time radix plan of a length L = 2^a 3^b 5^c 7^d
+// ------------------------------------------------------------------------
+struct RadixList {
+    int n;
+    int r[24];
+};
+
+__host__ __device__ constexpr RadixList make_radix_list(int L) {
+    RadixList s = {0, {0}};
+    int x = L, p2 = 0, n3 = 0, n5 = 0, n7 = 0;
+    while (x % 2 == 0) { x /= 2; ++p2; }
+    while (x % 3 == 0) { x /= 3; ++n3; }
+    while (x % 5 == 0) { x /= 5; ++n5; }
+    while (x % 7 == 0) { x /= 7; ++n7; }
+    // largest radix first: it fixes T = L / r[0] threads per instance
+    while (p2 >= 4) { s.r[s.n++] = 16; p2 -= 4; }
+    if (p2 > 0) s.r[s.n++] = 1 << p2;
+    for (int k = 0; k < n7; ++k) s.r[s.n++] = 7;
+    for (int k = 0; k < n5; ++k) s.r[s.n++] = 5;
+    for (int k = 0; k < n3; ++k) s.r[s.n++] = 3;
+    if (s.n == 0) s.r[s.n++] = 1;
+    // move the largest radix to the front (e.g. [4, 5, 5] -> [5, 4, 5])
+    int imax = 0;
+    for (int k = 1; k < s.n; ++k)
+        if (s.r[k] > s.r[imax]) imax = k;
+    int t = s.r[0]; s.r[0] = s.r[imax]; s.r[imax] = t;
+    return s;
+}
+
+__host__ __device__ constexpr bool is_supported_length(int L) {
+    int x = L;
+    if (x < 1) return false;
+    while (x % 2 == 0) x /= 2;
+    while (x % 3 == 0) x /= 3;
+    while (x % 5 == 0) x /= 5;
+    while (x % 7 == 0) x /= 7;
+    return x == 1;
+}
+
+template <int L> struct Plan {
+    static constexpr RadixList RL = make_radix_list(L);
+    static constexpr int NSTAGE = RL.n;
+    static constexpr int R0 = RL.r[0];
+    static constexpr int T = L / R0;
+    static constexpr int radix(int s) { return RL.r[s]; }
+    static constexpr int ns(int s) {
+        int p = 1;
+        for (int k = 0; k < s; ++k) p *= RL.r[k];
+        return p;
+    }
+    static constexpr int nb(int s) { return L / RL.r[s]; }
+    static constexpr int q(int s) { return (L / RL.r[s] + T - 1) / T; }
+    static constexpr int vmax() {
+        int m = 0;
+        for (int s = 0; s < RL.n; ++s)
+            if (q(s) * RL.r[s] > m) m = q(s) * RL.r[s];
+        return m;
+    }
+};
+
+// LDS layout of one instance: one padding slot every 16 complex values, and
+// an odd row pitch so that lanes walking adjacent instances hit distinct banks.
+RK_HD constexpr int lds_pad(int p) { return p + (p >> 4); }
+template <int L> __host__ __device__ constexpr int lds_pitch() {
+    int s = lds_pad(L - 1) + 1;
+    return (s % 2 == 0) ? s + 1 : s;
+}
+
+// ------------------------------------------------------------------------
+// per-instance coordinates
+// ------------------------------------------------------------------------
+struct Inst {
+    rk_u64 o, i, l;
+    rk_u64 jpass;    // h*NS + l: axis position of input element 0
+    rk_u64 kbase;    // h*NS*L + l: axis position of output element 0
+    rk_i64 in_base, out_base;
+    bool valid;
+};
+
+template <int L>
+__device__ __forceinline__ Inst decompose(const rk_pass_args& a, rk_u64 inst) {
+    Inst r;
+    r.valid = inst < a.n_inst;
+    rk_u64 x = r.valid ? inst : 0;
+    rk_u64 i = 0, l = 0, h = 0;
+    if (a.dim_I > 1) { i = x % a.dim_I; x /= a.dim_I; }
+    if (a.dim_NS > 1) { l = x % a.dim_NS; x /= a.dim_NS; }
+    if (a.dim_H > 1) { h = x % a.dim_H; x /= a.dim_H; }
+    r.o = x;
+    r.i = i;
+    r.l = l;
+    r.jpass = h * a.dim_NS + l;
+    r.kbase = h * a.dim_NS * (rk_u64)L + l;
+    r.in_base = a.in_off + (rk_i64)x * a.in_so + (rk_i64)i * a.in_si;
+    r.out_base = a.out_off + (rk_i64)x * a.out_so + (rk_i64)i * a.out_si;
+    return r;
+}
+
+// W_N^e for 0 <= e < N from two FP64 tables; the product is formed in FP64
+// and rounded to float once.
+__device__ __forceinline__ float2 twiddle_n(const rk_pass_args& a, rk_u64 e) {
+    double2 h = a.tw_hi[e >> a.tw_shift];
+    double2 l = a.tw_lo[e & a.tw_mask];
+    return make_float2((float)(h.x * l.x - h.y * l.y), (float)(h.x * l.y + h.y * l.x));
+}
+
+// Plain strided element access: the identity IO policy.
+// An IO policy is an object with load(a, o, m, i, addr) and
+// store(a, o, k, i, addr, v): o / i are the flat indices over the dims before /
+// after the transformed axis, m / k the axis position, addr the plain strided
+// element address.  Fused transformations (generated at run time) use o, m, i.
+struct PlainIO {
+    __device__ __forceinline__ float2 load(const rk_pass_args& a, rk_u64 /*o*/, rk_u64 /*m*/, rk_u64 /*i*/,
+                                           rk_i64 addr) const {
+        return a.in[addr];
+    }
+    __device__ __forceinline__ void store(const rk_pass_args& a, rk_u64 /*o*/, rk_u64 /*k*/, rk_u64 /*i*/,
+                                          rk_i64 addr, float2 v) const {
+        a.out[addr] = v;
+    }
+};
+
+// Compile-time pass variant: which pre-/post-ops the pass applies and whether
+// it is the single-kernel Bluestein convolution.  Keeping them out of the
+// run-time arguments keeps plain passes free of table loads (and registers).
+#define RK_VAR_PRE_NONE 0
+#define RK_VAR_PRE_TWIDDLE 1
+#define RK_VAR_PRE_CHIRP 2
+#define RK_VAR_POST_NONE 0
+#define RK_VAR_POST_KMUL_CONJ 1
+#define RK_VAR_POST_CONJ_CHIRP 2
+#define RK_VARIANT(pre, post, blue) ((pre) | ((post) << 2) | ((blue) << 4))
+template <int V> struct Variant {
+    static constexpr int PRE = V & 3;
+    static constexpr int POST = (V >> 2) & 3;
+    static constexpr bool BLUE = ((V >> 4) & 1) != 0;
+};
+
+template <int L, int V, class IO>
+__device__ __forceinline__ float2 load_elem(const rk_pass_args& a, const IO& io, const Inst& ic, int p) {
+    constexpr int PRE = Variant<V>::PRE;
+    const rk_u64 m = ic.jpass + (rk_u64)p * a.pos_mult;
+    float2 x = make_float2(0.0f, 0.0f);
+    if (ic.valid && m < a.n_valid_in) {
+        x = io.load(a, ic.o, m, ic.i, ic.in_base + (rk_i64)m * a.in_sa);
+        x.y *= a.in_sign;
+        if constexpr (PRE == RK_VAR_PRE_CHIRP) x = c_mul(x, a.chirp[m]);
+        if constexpr (PRE == RK_VAR_PRE_TWIDDLE) x = c_mul(x, twiddle_n(a, (rk_u64)p * ic.l * a.tw_mult));
+    }
+    return x;
+}
+
+template <int L, int V, class IO>
+__device__ __forceinline__ void store_elem(const rk_pass_args& a, const IO& io, const Inst& ic, int k, float2 y) {
+    constexpr int POST = Variant<V>::POST;
+    const rk_u64 kabs = ic.kbase + (rk_u64)k * a.dim_NS;
+    if (!ic.valid || kabs >= a.n_valid_out) return;
+    if constexpr (POST == RK_VAR_POST_KMUL_CONJ) y = c_conj(c_mul(y, a.bspec[kabs]));
+    if constexpr (POST == RK_VAR_POST_CONJ_CHIRP) y = c_mul(c_conj(y), a.chirp[kabs]);
+    y.y *= a.out_sign;
+    y = c_scale(y, a.out_scale);
+    io.store(a, ic.o, kabs, ic.i, ic.out_base + (rk_i64)kabs * a.out_sa, y);
+}
+
+// ------------------------------------------------------------------------
+// the Stockham stages of one tile
+// ------------------------------------------------------------------------
+template <int L, int S, int VM>
+__device__ __forceinline__ void run_stage(float2 (&v)[VM], float2* my, int t, const float2* twl,
+                                          bool from_lds) {
+    using P = Plan<L>;
+    constexpr int R = P::radix(S);
+    constexpr int NS = P::ns(S);
+    constexpr int NB = P::nb(S);
+    constexpr int Q = P::q(S);
+    constexpr int T = P::T;
+    constexpr bool EVEN = (NB % T) == 0;
+
+    if constexpr (S > 0) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int j = t + q * T;
+            if (EVEN || j < NB) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[q * R + r] = my[lds_pad(j + r * NB)];
+                const int m = j % NS;
+#pragma unroll
+                for (int r = 1; r < R; ++r)
+                    v[q * R + r] = c_mul(v[q * R + r], twl[(r * m) * (L / (NS * R))]);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int j = t + q * T;
+        if (EVEN || j < NB) dft<R>(&v[q * R]);
+    }
+    if constexpr (S + 1 < P::NSTAGE) {
+        if (S > 0 || from_lds) __syncthreads();   // every read of the tile is done
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int j = t + q * T;
+            if (EVEN || j < NB) {
+                const int base = (j / NS) * NS * R + (j % NS);
+#pragma unroll
+                for (int r = 0; r < R; ++r) my[lds_pad(base + r * NS)] = v[q * R + r];
+            }
+        }
+        __syncthreads();
+        run_stage<L, S + 1, VM>(v, my, t, twl, true);
+    }
+}
+
+// Position of the value held in v[q*R + r] after the last stage.
+template <int L>
+__device__ __forceinline__ int out_pos(int t, int q, int r) {
+    using P = Plan<L>;
+    constexpr int S = P::NSTAGE - 1;
+    return t + q * P::T + r * P::nb(S);
+}
+
+template <int L>
+__device__ __forceinline__ bool out_valid(int t, int q) {
+    using P = Plan<L>;
+    constexpr int S = P::NSTAGE - 1;
+    return (P::nb(S) % P::T) == 0 || (t + q * P::T) < P::nb(S);
+}
+
+// ------------------------------------------------------------------------
+// the pass kernel body
+// ------------------------------------------------------------------------
+template <int L, int C, int V, class IO>
+__device__ __forceinline__ void fft_pass_body(const rk_pass_args& a, const IO& io) {
+    using P = Plan<L>;
+    constexpr int T = P::T;
+    constexpr int NT = C * T;
+    constexpr int R0 = P::R0;
+    constexpr int LP = lds_pitch<L>();
+    constexpr int VM = P::vmax();
+    constexpr int SL = P::NSTAGE - 1;
+    constexpr int RL = P::radix(SL);
+    constexpr int QL = P::q(SL);
+    constexpr bool BLUE = Variant<V>::BLUE;
+    __shared__ float2 lds[C * LP];
+
+    const int tid = threadIdx.x;
+    const int c = tid / T;
+    const int t = tid - c * T;
+    float2* my = lds + c * LP;
+    // column mapping: lanes across instances
+    const int cc = tid % C;
+    const int pc = tid / C;
+
+    // One tile per work-group: a loop over tiles would let the compiler keep
+    // every per-thread address of the stages live across iterations.
+    {
+        const rk_u64 tile = (rk_u64)blockIdx.x + a.tile_base;
+        if (tile >= a.n_tiles) return;
+        const rk_u64 inst0 = tile * (rk_u64)C;
+        const Inst ic = decompose<L>(a, inst0 + c);
+        float2 v[VM];
+
+        // ---- stage-0 inputs: element p = t + r*T ----
+        bool from_lds = false;
+        if (a.load_mode == RK_MODE_DIRECT) {
+#pragma unroll
+            for (int r = 0; r < R0; ++r) v[r] = load_elem<L, V, IO>(a, io, ic, t + r * T);
+        } else {
+            if (a.load_mode == RK_MODE_COLUMN) {
+                const Inst icc = decompose<L>(a, inst0 + cc);
+#pragma unroll
+                for (int it = 0; it < R0; ++it) {
+                    const int p = pc + it * T;
+                    lds[cc * LP + lds_pad(p)] = load_elem<L, V, IO>(a, io, icc, p);
+                }
+            } else {   // FLAT: instance-linear block of C*L elements, no pre-ops
+                const rk_i64 base = a.in_off + (rk_i64)inst0 * L;
+#pragma unroll
+                for (int it = 0; it < R0; ++it) {
+                    const int e = tid + it * NT;
+                    const int ce = e / L, p = e - ce * L;
+                    float2 x = make_float2(0.0f, 0.0f);
+                    if (inst0 + ce < a.n_inst) {
+                        x = io.load(a, inst0 + ce, p, 0, base + e);
+                        x.y *= a.in_sign;
+                    }
+                    lds[ce * LP + lds_pad(p)] = x;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < R0; ++r) v[r] = my[lds_pad(t + r * T)];
+            from_lds = true;
+        }
+
+        run_stage<L, 0, VM>(v, my, t, a.twl, from_lds);
+
+        if constexpr (BLUE) {
+            // A = FFT(chirp * x): multiply by the kernel spectrum, conjugate,
+            // and transform again (the inverse DFT as conj(FFT(conj .))).
+#pragma unroll
+            for (int q = 0; q < QL; ++q)
+                if (out_valid<L>(t, q)) {
+#pragma unroll
+                    for (int r = 0; r < RL; ++r) {
+                        const int k = out_pos<L>(t, q, r);
+                        v[q * RL + r] = c_conj(c_mul(v[q * RL + r], a.bspec[k]));
+                    }
+                }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < QL; ++q)
+                if (out_valid<L>(t, q)) {
+#pragma unroll
+                    for (int r = 0; r < RL; ++r) my[lds_pad(out_pos<L>(t, q, r))] = v[q * RL + r];
+                }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < R0; ++r) v[r] = my[lds_pad(t + r * T)];
+            run_stage<L, 0, VM>(v, my, t, a.twl, true);
+        }
+
+        // ---- store: output element k ----
+        if (a.store_mode == RK_MODE_DIRECT) {
+#pragma unroll
+            for (int q = 0; q < QL; ++q)
+                if (out_valid<L>(t, q)) {
+#pragma unroll
+                    for (int r = 0; r < RL; ++r)
+                        store_elem<L, V, IO>(a, io, ic, out_pos<L>(t, q, r), v[q * RL + r]);
+                }
+        } else {
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < QL; ++q)
+                if (out_valid<L>(t, q)) {
+#pragma unroll
+                    for (int r = 0; r < RL; ++r) my[lds_pad(out_pos<L>(t, q, r))] = v[q * RL + r];
+                }
+            __syncthreads();
+            if (a.store_mode == RK_MODE_COLUMN) {
+                const Inst icc = decompose<L>(a, inst0 + cc);
+#pragma unroll
+                for (int it = 0; it < R0; ++it) {
+                    const int k = pc + it * T;
+                    store_elem<L, V, IO>(a, io, icc, k, lds[cc * LP + lds_pad(k)]);
+                }
+            } else {   // FLAT, no post-ops beyond sign and scale
+                const rk_i64 base = a.out_off + (rk_i64)inst0 * L;
+#pragma unroll
+                for (int it = 0; it < R0; ++it) {
+                    const int e = tid + it * NT;
+                    const int ce = e / L, k = e - ce * L;
+                    if (inst0 + ce < a.n_inst) {
+                        float2 y = lds[ce * LP + lds_pad(k)];
+                        y.y *= a.out_sign;
+                        y = c_scale(y, a.out_scale);
+                        io.store(a, inst0 + ce, k, 0, base + e, y);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Threads per work-group of fft_pass_body<L, C, ...>.
+template <int L, int C> __host__ __device__ constexpr int pass_threads() { return C * Plan<L>::T; }
+
+}   // namespace rkfft
+
+#endif

@@ -1,0 +1,831 @@
+// rk_fft_plan.cpp — the FFT planner and executor behind rk_fft_plan_create /
+// rk_fft_execute.
+//
+// What it replaces in the reference:
+//   get_fft_kernels / get_fft_1d_kernels (reikna/fft/fft.py:487-584): the
+//     per-axis kernel chain.  Here an axis of smooth length N (2^a 3^b 5^c 7^d)
+//     is one pass when N <= kSingleMax and otherwise two or more Stockham
+//     passes whose lengths multiply to N; a non-smooth axis goes through
+//     Bluestein's chirp-z transform over a smooth M >= 2N-1 (the reference uses
+//     M = 2 * bounding_power_of_2(N), fft.py:551-561).
+//   _build_limited_plan buffer routing (fft.py:664-721): the first pass reads
+//     `in`, the last writes `out`, intermediate passes ping-pong between two
+//     workspace buffers.
+//   get_kweights (fft.py:216-236): the Bluestein kernel spectrum, here an FP64
+//     host FFT rounded once to complex64.
+//   xweight (fft.mako:196-202): the chirp exp(-i pi (n^2 mod 2N)/N) with the
+//     exact integer n^2 mod 2N, here evaluated in FP64 and rounded once.
+//   the run-time `inverse` flag and 1/N normalisation (fft.mako:741-747).
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <memory>
+
+#include "fft_device.hpp"
+#include "rk_internal.hpp"
+
+using rk::fail;
+
+namespace rk {
+
+static constexpr int64_t kSingleMax = 16384;   // longest single-pass DFT
+static constexpr int64_t kFactorMax = 4096;    // longest DFT of one pass in a multi-pass chain
+
+enum BufId { BUF_IN = 0, BUF_OUT = 1, BUF_WS0 = 2, BUF_WS1 = 3 };
+
+bool is_smooth(int64_t n) {
+    if (n < 1) return false;
+    for (int p : {2, 3, 5, 7})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+// ---------------------------------------------------------------------------
+// host mirrors of the compile-time geometry in fft_device.hpp
+// ---------------------------------------------------------------------------
+int threads_per_instance(int L) {
+    rkfft::RadixList rl = rkfft::make_radix_list(L);
+    return L / rl.r[0];
+}
+
+int lds_pitch_host(int L) {
+    int s = rkfft::lds_pad(L - 1) + 1;
+    return (s % 2 == 0) ? s + 1 : s;
+}
+
+int tile_c_row(int L) {
+    int T = threads_per_instance(L);
+    int C = std::max(1, 256 / T);
+    while (C > 1 && (int64_t)C * lds_pitch_host(L) * 8 > 64 * 1024) C /= 2;
+    return C;
+}
+
+int tile_c_col(int L) {
+    int T = threads_per_instance(L);
+    int C = std::max(16, (tile_c_row(L) / 16) * 16);
+    while (C > 1 && ((int64_t)C * T > 1024 || (int64_t)C * lds_pitch_host(L) * 8 > 159 * 1024)) C /= 2;
+    return C;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 host math for the tables
+// ---------------------------------------------------------------------------
+using cd = std::complex<double>;
+
+// exp(-2 pi i e / n), e reduced exactly modulo n first
+static cd unit_root(int64_t e, int64_t n) {
+    e %= n;
+    if (e < 0) e += n;
+    // use the symmetric argument (-n/2, n/2] for a smaller FP64 argument
+    int64_t s = (2 * e > n) ? e - n : e;
+    double ang = -2.0 * M_PI * (double)s / (double)n;
+    return cd(std::cos(ang), std::sin(ang));
+}
+
+// Forward DFT in FP64 of a smooth length (recursive mixed radix), used only
+// for the Bluestein kernel spectrum.
+static void fft_host(std::vector<cd>& x) {
+    const int64_t n = (int64_t)x.size();
+    if (n <= 1) return;
+    int64_t p = 0;
+    for (int q : {4, 2, 3, 5, 7})
+        if (n % q == 0) { p = q; break; }
+    if (p == 0) {   // not smooth: direct O(n^2) (never hit by the planner)
+        std::vector<cd> y(n);
+        for (int64_t k = 0; k < n; ++k) {
+            cd s = 0;
+            for (int64_t j = 0; j < n; ++j) s += x[j] * unit_root(j * k % n, n);
+            y[k] = s;
+        }
+        x.swap(y);
+        return;
+    }
+    const int64_t m = n / p;
+    std::vector<std::vector<cd>> sub(p, std::vector<cd>(m));
+    for (int64_t j = 0; j < m; ++j)
+        for (int64_t r = 0; r < p; ++r) sub[r][j] = x[j * p + r];
+    for (auto& s : sub) fft_host(s);
+    for (int64_t k = 0; k < m; ++k)
+        for (int64_t q = 0; q < p; ++q) {
+            cd s = 0;
+            const int64_t kk = k + q * m;
+            for (int64_t r = 0; r < p; ++r) s += sub[r][k] * unit_root(r * kk % n, n);
+            x[kk] = s;
+        }
+}
+
+// ---------------------------------------------------------------------------
+// plan structures
+// ---------------------------------------------------------------------------
+struct Layout {
+    int ndim;
+    int64_t shape[RK_MAX_DIMS];
+    int64_t strides[RK_MAX_DIMS];
+    int64_t offset;
+};
+
+static Layout contiguous_layout(int ndim, const int64_t* shape) {
+    Layout l;
+    l.ndim = ndim;
+    l.offset = 0;
+    int64_t s = 1;
+    for (int d = ndim - 1; d >= 0; --d) {
+        l.shape[d] = shape[d];
+        l.strides[d] = s;
+        s *= shape[d];
+    }
+    return l;
+}
+
+struct HostLoop {   // batch dims that could not be folded into the kernel's O / I
+    int64_t n, in_str, out_str;
+};
+
+struct PassPlan {
+    rk_pass_args args;
+    int src, dst;
+    int L, C, variant, threads, lds_bytes;
+    const void* aot = nullptr;      // ahead-of-time kernel
+    hipFunction_t jit = nullptr;    // run-time compiled kernel
+    bool user_kernel = false;       // replaced with rk_fft_plan_set_pass_kernel
+    bool is_first = false, is_last = false;
+    uint64_t grid = 0;
+    int axis = 0;
+    int kind = 0;
+    int64_t twl_off = -1, twhi_off = -1, twlo_off = -1, chirp_off = -1, bspec_off = -1;
+    std::vector<HostLoop> loops;
+};
+
+struct TableBuilder {
+    std::vector<unsigned char> bytes;
+    std::map<std::string, int64_t> index;
+    int64_t add(const std::string& key, const void* data, size_t n) {
+        auto it = index.find(key);
+        if (it != index.end()) return it->second;
+        size_t off = (bytes.size() + 255) & ~size_t(255);
+        bytes.resize(off + n);
+        memcpy(bytes.data() + off, data, n);
+        index[key] = (int64_t)off;
+        return (int64_t)off;
+    }
+};
+
+}  // namespace rk
+
+struct rk_fft_plan {
+    rk_fft_desc desc;
+    std::vector<rk::PassPlan> passes;
+    void* tables = nullptr;
+    size_t ws_bytes = 0, ws1_offset = 0;
+    double norm = 1.0;
+    bool trivial = false;
+    rk::CopyArgs copy;
+    bool timing = false;
+    std::vector<hipEvent_t> events;   // 2 per pass when timing
+    int device = 0;
+};
+
+namespace rk {
+
+// ---------------------------------------------------------------------------
+// planning helpers
+// ---------------------------------------------------------------------------
+
+// Split a smooth n into pass lengths, each <= kFactorMax, as balanced as possible.
+static bool split_factors(int64_t n, std::vector<int64_t>& out) {
+    out.clear();
+    if (n <= kFactorMax) {
+        out.push_back(n);
+        return true;
+    }
+    // two factors: the divisor d <= sqrt(n) that is smooth and maximises min(d, n/d)
+    int64_t best = 0;
+    for (int64_t d = 2; d * d <= n; ++d)
+        if (n % d == 0 && is_smooth(d) && n / d <= kFactorMax) best = d;
+    if (best) {
+        out.push_back(best);
+        out.push_back(n / best);
+        return true;
+    }
+    // otherwise peel the largest factor <= kFactorMax that keeps a valid rest
+    for (int64_t d = kFactorMax; d >= 2; --d)
+        if (n % d == 0 && is_smooth(d)) {
+            std::vector<int64_t> rest;
+            if (!split_factors(n / d, rest)) continue;
+            out.push_back(d);
+            out.insert(out.end(), rest.begin(), rest.end());
+            return true;
+        }
+    return false;
+}
+
+// Smallest M >= 2n-1 of the form 2^k, 3*2^k or 5*2^k.
+static int64_t bluestein_length(int64_t n) {
+    int64_t need = 2 * n - 1, best = 0;
+    for (int64_t base : {1, 3, 5}) {
+        int64_t m = base;
+        while (m < need) m *= 2;
+        if (!best || m < best) best = m;
+    }
+    return best;
+}
+
+struct AxisPass {
+    int L;
+    int64_t Nax;      // axis length the Stockham step works on (N or M)
+    int64_t NS, H;
+    bool blue_single;
+    uint32_t pre_ops, post_ops;
+    int64_t n_valid_in, n_valid_out;
+    int64_t in_len, out_len;   // axis length in the src / dst layouts
+};
+
+// The passes of one axis of length n.
+static int axis_passes(int64_t n, uint32_t flags, std::vector<AxisPass>& out) {
+    out.clear();
+    const bool smooth = is_smooth(n) && !(flags & RK_FFT_FORCE_BLUESTEIN);
+    auto stockham_chain = [&](int64_t N, const std::vector<int64_t>& f, int64_t in_len, int64_t out_len,
+                              uint32_t first_pre, uint32_t last_post, int64_t valid_in, int64_t valid_out) {
+        int64_t ns = 1;
+        for (size_t p = 0; p < f.size(); ++p) {
+            AxisPass ap;
+            ap.L = (int)f[p];
+            ap.Nax = N;
+            ap.NS = ns;
+            ap.H = N / (ns * f[p]);
+            ap.blue_single = false;
+            ap.pre_ops = (p == 0 ? first_pre : 0u) | (ns > 1 ? RK_PRE_TWIDDLE : 0u);
+            ap.post_ops = (p + 1 == f.size()) ? last_post : 0u;
+            ap.n_valid_in = (p == 0) ? valid_in : N;
+            ap.n_valid_out = (p + 1 == f.size()) ? valid_out : N;
+            ap.in_len = (p == 0) ? in_len : N;
+            ap.out_len = (p + 1 == f.size()) ? out_len : N;
+            out.push_back(ap);
+            ns *= f[p];
+        }
+    };
+    if (smooth) {
+        std::vector<int64_t> f;
+        if (n <= kSingleMax && !(flags & RK_FFT_FORCE_MULTIPASS)) {
+            f.push_back(n);
+        } else if (!split_factors(n, f)) {
+            return fail(RK_ERR_UNSUPPORTED, "cannot split axis length %lld into passes", (long long)n);
+        }
+        if ((flags & RK_FFT_FORCE_MULTIPASS) && f.size() == 1 && n > 1) {
+            // split a single-pass length into two passes for testing the chain
+            int64_t d = 0;
+            for (int64_t k = 2; k * k <= n; ++k)
+                if (n % k == 0) d = k;
+            if (d) f = {d, n / d};
+        }
+        stockham_chain(n, f, n, n, 0u, 0u, n, n);
+        return RK_OK;
+    }
+    const int64_t m = bluestein_length(n);
+    if (m <= kSingleMax && !(flags & RK_FFT_FORCE_MULTIPASS)) {
+        AxisPass ap;
+        ap.L = (int)m;
+        ap.Nax = m;
+        ap.NS = 1;
+        ap.H = 1;
+        ap.blue_single = true;
+        ap.pre_ops = RK_PRE_CHIRP;
+        ap.post_ops = RK_POST_CONJ | RK_POST_CHIRP;
+        ap.n_valid_in = n;
+        ap.n_valid_out = n;
+        ap.in_len = n;
+        ap.out_len = n;
+        out.push_back(ap);
+        return RK_OK;
+    }
+    std::vector<int64_t> f;
+    if (!split_factors(m, f)) return fail(RK_ERR_UNSUPPORTED, "cannot split Bluestein length %lld", (long long)m);
+    if (f.size() == 1) {   // forced multi-pass of a short M
+        int64_t d = 0;
+        for (int64_t k = 2; k * k <= m; ++k)
+            if (m % k == 0) d = k;
+        if (d) f = {d, m / d};
+    }
+    // forward M-point transform of chirp * x (zero padded), then multiply by
+    // the kernel spectrum and conjugate ...
+    stockham_chain(m, f, n, m, RK_PRE_CHIRP, RK_POST_KMUL_CONJ, n, m);
+    // ... forward transform again, conjugate, chirp, truncate to n.
+    std::vector<AxisPass> second;
+    std::swap(second, out);
+    stockham_chain(m, f, m, n, 0u, RK_POST_CONJ | RK_POST_CHIRP, m, n);
+    second.insert(second.end(), out.begin(), out.end());
+    out.swap(second);
+    return RK_OK;
+}
+
+// Fold the batch dims (all dims but `axis`) of the src and dst layouts into
+// O (outer, before the axis) and I (inner, after the axis) groups the kernel
+// walks with one stride each; what does not fold becomes host loops.
+static void fold_batch(const Layout& src, const Layout& dst, int axis, int64_t& O, int64_t& so_in, int64_t& so_out,
+                       int64_t& I, int64_t& si_in, int64_t& si_out, std::vector<HostLoop>& loops) {
+    auto fold = [&](int d0, int d1, std::vector<HostLoop>& groups) {
+        // d0..d1-1, outermost first; returns groups innermost last
+        for (int d = d0; d < d1; ++d) {
+            if (src.shape[d] == 1) continue;
+            HostLoop g{src.shape[d], src.strides[d], dst.strides[d]};
+            if (!groups.empty()) {
+                HostLoop& prev = groups.back();
+                if (prev.in_str == g.in_str * g.n && prev.out_str == g.out_str * g.n) {
+                    prev.n *= g.n;
+                    prev.in_str = g.in_str;
+                    prev.out_str = g.out_str;
+                    continue;
+                }
+            }
+            groups.push_back(g);
+        }
+    };
+    std::vector<HostLoop> outer, inner;
+    fold(0, axis, outer);
+    fold(axis + 1, src.ndim, inner);
+    O = 1; so_in = 0; so_out = 0;
+    I = 1; si_in = 0; si_out = 0;
+    if (!inner.empty()) {
+        I = inner.back().n;
+        si_in = inner.back().in_str;
+        si_out = inner.back().out_str;
+        inner.pop_back();
+    }
+    if (!outer.empty()) {
+        O = outer.back().n;
+        so_in = outer.back().in_str;
+        so_out = outer.back().out_str;
+        outer.pop_back();
+    }
+    loops.clear();
+    for (auto& g : outer) loops.push_back(g);
+    for (auto& g : inner) loops.push_back(g);
+}
+
+static int find_kernel(PassPlan& pp, uint32_t flags) {
+    for (const KernelEntry& e : catalog())
+        if (e.L == pp.L && e.C == pp.C && e.variant == pp.variant) {
+            pp.aot = e.func;
+            pp.threads = e.threads;
+            pp.lds_bytes = e.lds_bytes;
+            return RK_OK;
+        }
+    if (flags & RK_FFT_NO_JIT)
+        return fail(RK_ERR_NO_KERNEL, "no precompiled kernel for L=%d C=%d variant=%d and run-time compilation is off",
+                    pp.L, pp.C, pp.variant);
+    int rc = jit_pass_kernel(pp.L, pp.C, pp.variant, &pp.jit, &pp.threads, &pp.lds_bytes);
+    if (rc != RK_OK) return rc;
+    return RK_OK;
+}
+
+static int build_tables(rk_fft_plan* plan, TableBuilder& tb, PassPlan& pp, const AxisPass& ap, int64_t nreal) {
+    // in-pass stage twiddles W_L^e
+    {
+        std::vector<float2> w(pp.L);
+        for (int e = 0; e < pp.L; ++e) {
+            cd z = unit_root(e, pp.L);
+            w[e] = make_float2((float)z.real(), (float)z.imag());
+        }
+        pp.twl_off = tb.add("twl" + std::to_string(pp.L), w.data(), w.size() * sizeof(float2));
+    }
+    if (ap.pre_ops & RK_PRE_TWIDDLE) {
+        int64_t N = ap.Nax;
+        int shift = 0;
+        while ((int64_t(1) << (2 * shift)) < N) ++shift;   // 2^shift >= sqrt(N)
+        const int64_t lo_n = int64_t(1) << shift;
+        const int64_t hi_n = (N + lo_n - 1) / lo_n;
+        std::vector<double2> hi(hi_n), lo(lo_n);
+        for (int64_t k = 0; k < hi_n; ++k) {
+            cd z = unit_root(k * lo_n, N);
+            hi[k] = make_double2(z.real(), z.imag());
+        }
+        for (int64_t k = 0; k < lo_n; ++k) {
+            cd z = unit_root(k, N);
+            lo[k] = make_double2(z.real(), z.imag());
+        }
+        pp.twhi_off = tb.add("twhi" + std::to_string(N), hi.data(), hi.size() * sizeof(double2));
+        pp.twlo_off = tb.add("twlo" + std::to_string(N), lo.data(), lo.size() * sizeof(double2));
+        pp.args.tw_shift = (rk_u32)shift;
+        pp.args.tw_mask = (rk_u64)(lo_n - 1);
+    }
+    const bool needs_chirp = (ap.pre_ops & RK_PRE_CHIRP) || (ap.post_ops & RK_POST_CHIRP);
+    if (needs_chirp) {
+        // exp(-i pi (n^2 mod 2N) / N): the phase index is an exact integer
+        std::vector<float2> ch(nreal);
+        const int64_t two_n = 2 * nreal;
+        for (int64_t k = 0; k < nreal; ++k) {
+            const unsigned __int128 sq = (unsigned __int128)k * (unsigned __int128)k;
+            const int64_t ph = (int64_t)(sq % (unsigned __int128)two_n);
+            cd z = unit_root(ph, two_n);   // exp(-2 pi i ph / 2N) = exp(-i pi ph / N)
+            ch[k] = make_float2((float)z.real(), (float)z.imag());
+        }
+        pp.chirp_off = tb.add("chirp" + std::to_string(nreal), ch.data(), ch.size() * sizeof(float2));
+    }
+    const bool needs_bspec = ap.blue_single || (ap.post_ops & RK_POST_KMUL_CONJ);
+    if (needs_bspec) {
+        const int64_t m = ap.Nax, two_n = 2 * nreal;
+        std::vector<cd> b(m, cd(0, 0));
+        for (int64_t k = 0; k < nreal; ++k) {
+            const unsigned __int128 sq = (unsigned __int128)k * (unsigned __int128)k;
+            const int64_t ph = (int64_t)(sq % (unsigned __int128)two_n);
+            cd c = std::conj(unit_root(ph, two_n));   // conj(chirp[k]) = exp(+i pi k^2 / N)
+            b[k] = c;
+            if (k > 0) b[m - k] = c;
+        }
+        fft_host(b);
+        std::vector<float2> bs(m);
+        for (int64_t k = 0; k < m; ++k) {
+            cd z = b[k] / (double)m;
+            bs[k] = make_float2((float)z.real(), (float)z.imag());
+        }
+        pp.bspec_off = tb.add("bspec" + std::to_string(nreal) + "_" + std::to_string(m), bs.data(),
+                              bs.size() * sizeof(float2));
+    }
+    (void)plan;
+    return RK_OK;
+}
+
+static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
+    if (!d || !out) return fail(RK_ERR_INVALID, "rk_fft_plan_create: null argument");
+    *out = nullptr;
+    if (d->dtype != RK_C64) return fail(RK_ERR_UNSUPPORTED, "only complex64 is supported");
+    if (d->ndim < 1 || d->ndim > RK_MAX_DIMS) return fail(RK_ERR_INVALID, "ndim must be in [1, %d]", RK_MAX_DIMS);
+    for (int k = 0; k < d->ndim; ++k)
+        if (d->shape[k] < 1) return fail(RK_ERR_INVALID, "shape[%d] = %lld must be >= 1", k, (long long)d->shape[k]);
+    if (d->naxes < 0 || d->naxes > d->ndim) return fail(RK_ERR_INVALID, "naxes out of range");
+    std::vector<int> axes;
+    if (d->naxes == 0) {
+        for (int k = 0; k < d->ndim; ++k) axes.push_back(k);
+    } else {
+        for (int k = 0; k < d->naxes; ++k) {
+            int a = d->axes[k];
+            if (a < 0 || a >= d->ndim) return fail(RK_ERR_INVALID, "axis %d out of range for ndim %d", a, d->ndim);
+            if (std::find(axes.begin(), axes.end(), a) != axes.end())
+                return fail(RK_ERR_INVALID, "axis %d repeated", a);
+            axes.push_back(a);
+        }
+    }
+
+    std::unique_ptr<rk_fft_plan> plan(new rk_fft_plan());
+    plan->desc = *d;
+    hipGetDevice(&plan->device);
+
+    Layout lin, lout;
+    lin.ndim = lout.ndim = d->ndim;
+    for (int k = 0; k < d->ndim; ++k) {
+        lin.shape[k] = lout.shape[k] = d->shape[k];
+        lin.strides[k] = d->in_strides[k];
+        lout.strides[k] = d->out_strides[k];
+    }
+    lin.offset = d->in_offset;
+    lout.offset = d->out_offset;
+
+    // axes with length > 1, innermost first (as the reference orders kernels)
+    std::vector<int> work;
+    for (auto it = axes.rbegin(); it != axes.rend(); ++it)
+        if (d->shape[*it] > 1) work.push_back(*it);
+
+    double norm = 1.0;
+    for (int a : work) norm /= (double)d->shape[a];
+    plan->norm = norm;
+
+    if (work.empty()) {
+        plan->trivial = true;
+        CopyArgs& c = plan->copy;
+        memset(&c, 0, sizeof(c));
+        c.ndim = d->ndim;
+        c.total = 1;
+        for (int k = 0; k < d->ndim; ++k) {
+            c.shape[k] = d->shape[k];
+            c.in_str[k] = d->in_strides[k];
+            c.out_str[k] = d->out_strides[k];
+            c.total *= (rk_u64)d->shape[k];
+        }
+        c.in_off = d->in_offset;
+        c.out_off = d->out_offset;
+        *out = plan.release();
+        return RK_OK;
+    }
+
+    TableBuilder tb;
+    int cur = BUF_IN;
+    int64_t cur_shape[RK_MAX_DIMS];
+    for (int k = 0; k < d->ndim; ++k) cur_shape[k] = d->shape[k];
+    size_t ws_elems = 0;
+    bool two_ws = false;
+
+    for (size_t wi = 0; wi < work.size(); ++wi) {
+        const int axis = work[wi];
+        const int64_t n = d->shape[axis];
+        std::vector<AxisPass> aps;
+        int rc = axis_passes(n, d->flags, aps);
+        if (rc != RK_OK) return rc;
+        if (aps.size() >= 3) two_ws = true;
+        int ws_next = BUF_WS0;
+        for (size_t pi = 0; pi < aps.size(); ++pi) {
+            const AxisPass& ap = aps[pi];
+            const bool axis_last = (pi + 1 == aps.size());
+            int dst;
+            const bool plan_last = axis_last && (wi + 1 == work.size());
+            if (axis_last && (plan_last || !(d->flags & RK_FFT_OUT_LAST_ONLY))) {
+                dst = BUF_OUT;
+            } else {
+                dst = ws_next;
+                if (dst == cur) dst = (dst == BUF_WS0) ? BUF_WS1 : BUF_WS0;
+                ws_next = (dst == BUF_WS0) ? BUF_WS1 : BUF_WS0;
+            }
+            // layouts of src and dst for this pass
+            int64_t s_shape[RK_MAX_DIMS], d_shape[RK_MAX_DIMS];
+            for (int k = 0; k < d->ndim; ++k) s_shape[k] = d_shape[k] = cur_shape[k];
+            s_shape[axis] = ap.in_len;
+            d_shape[axis] = ap.out_len;
+            Layout ls = (cur == BUF_IN) ? lin : (cur == BUF_OUT) ? lout : contiguous_layout(d->ndim, s_shape);
+            Layout ld = (dst == BUF_OUT) ? lout : contiguous_layout(d->ndim, d_shape);
+            if (dst == BUF_WS0 || dst == BUF_WS1) {
+                size_t e = 1;
+                for (int k = 0; k < d->ndim; ++k) e *= (size_t)d_shape[k];
+                ws_elems = std::max(ws_elems, e);
+            }
+
+            PassPlan pp;
+            memset(&pp.args, 0, sizeof(pp.args));
+            pp.src = cur;
+            pp.dst = dst;
+            pp.axis = axis;
+            pp.L = ap.L;
+            pp.kind = ap.blue_single ? 1 : 0;
+            if (ap.blue_single) {
+                pp.variant = VAR_BLUE;
+            } else {
+                int pre = (ap.pre_ops & RK_PRE_TWIDDLE) ? 1 : (ap.pre_ops & RK_PRE_CHIRP) ? 2 : 0;
+                int post = (ap.post_ops & RK_POST_KMUL_CONJ) ? 1 : (ap.post_ops & RK_POST_CHIRP) ? 2 : 0;
+                if ((ap.pre_ops & RK_PRE_TWIDDLE) && (ap.pre_ops & RK_PRE_CHIRP))
+                    return fail(RK_ERR_UNSUPPORTED, "internal: pass with both chirp and twiddle pre-ops");
+                pp.variant = pre | (post << 2);
+            }
+            rk_pass_args& a = pp.args;
+            int64_t O, so_in, so_out, I, si_in, si_out;
+            fold_batch(ls, ld, axis, O, so_in, so_out, I, si_in, si_out, pp.loops);
+            a.dim_I = (rk_u64)I;
+            a.dim_NS = (rk_u64)ap.NS;
+            a.dim_H = (rk_u64)ap.H;
+            a.n_inst = (rk_u64)(O * ap.H * ap.NS * I);
+            a.in_off = ls.offset;
+            a.out_off = ld.offset;
+            a.in_so = so_in;
+            a.out_so = so_out;
+            a.in_si = si_in;
+            a.out_si = si_out;
+            a.in_sa = ls.strides[axis];
+            a.out_sa = ld.strides[axis];
+            a.N = (rk_u64)ap.Nax;
+            a.pos_mult = (rk_u64)(ap.Nax / ap.L);
+            a.tw_mult = (rk_u64)(ap.Nax / (ap.NS * ap.L));
+            a.pre_ops = ap.pre_ops;
+            a.post_ops = ap.post_ops;
+            a.n_valid_in = (rk_u64)ap.n_valid_in;
+            a.n_valid_out = (rk_u64)ap.n_valid_out;
+            a.in_sign = 1.0f;
+            a.out_sign = 1.0f;
+            a.out_scale = 1.0f;
+
+            // global-memory walk of each side
+            const int64_t L = ap.L;
+            const int64_t pos_in = (int64_t)a.pos_mult * a.in_sa;
+            const int64_t inst_in = I > 1 ? si_in : (ap.NS > 1 || ap.H > 1) ? a.in_sa : so_in;
+            const bool flat_in = ap.H == 1 && ap.NS == 1 && I == 1 && a.in_sa == 1 && (O == 1 || so_in == L) &&
+                                 ap.pre_ops == 0 && ap.n_valid_in == ap.Nax && pp.loops.empty();
+            if (flat_in) a.load_mode = RK_MODE_FLAT;
+            else if (pos_in == 1) a.load_mode = RK_MODE_DIRECT;
+            else if (inst_in == 1) a.load_mode = RK_MODE_COLUMN;
+            else a.load_mode = RK_MODE_DIRECT;
+            const int64_t pos_out = ap.NS * a.out_sa;
+            const int64_t inst_out = I > 1 ? si_out : ap.NS > 1 ? a.out_sa : ap.H > 1 ? L * a.out_sa : so_out;
+            const bool flat_out = ap.NS == 1 && I == 1 && a.out_sa == 1 &&
+                                  (ap.H == 1 ? (O == 1 || so_out == L) : (O == 1 || so_out == ap.Nax)) &&
+                                  ap.post_ops == 0 && ap.n_valid_out == ap.Nax && pp.loops.empty();
+            if (flat_out) a.store_mode = RK_MODE_FLAT;
+            else if (pos_out == 1) a.store_mode = RK_MODE_DIRECT;
+            else if (inst_out == 1) a.store_mode = RK_MODE_COLUMN;
+            else a.store_mode = RK_MODE_DIRECT;
+            const bool column = a.load_mode == RK_MODE_COLUMN || a.store_mode == RK_MODE_COLUMN;
+            pp.C = column ? tile_c_col(ap.L) : tile_c_row(ap.L);
+
+            rc = find_kernel(pp, d->flags);
+            if (rc != RK_OK) return rc;
+            a.n_tiles = (a.n_inst + (rk_u64)pp.C - 1) / (rk_u64)pp.C;
+            const uint64_t max_grid = (uint64_t(1) << 31) / (uint64_t)pp.threads;
+            pp.grid = std::min<uint64_t>(a.n_tiles, max_grid);
+            rc = build_tables(plan.get(), tb, pp, ap, n);
+            if (rc != RK_OK) return rc;
+            plan->passes.push_back(pp);
+            cur = dst;
+            cur_shape[axis] = ap.out_len;
+        }
+    }
+    plan->passes.front().is_first = true;
+    plan->passes.back().is_last = true;
+
+    plan->ws_bytes = 0;
+    if (ws_elems) {
+        const size_t one = ((ws_elems * sizeof(float2)) + 255) & ~size_t(255);
+        plan->ws1_offset = one;
+        plan->ws_bytes = two_ws ? 2 * one : one;
+        // a plan whose passes use WS1 without ever needing two at once still
+        // gets both slots
+        for (auto& p : plan->passes)
+            if (p.src == BUF_WS1 || p.dst == BUF_WS1) plan->ws_bytes = 2 * one;
+    }
+    if (!tb.bytes.empty()) {
+        RK_HIP(hipMalloc(&plan->tables, tb.bytes.size()));
+        RK_HIP(hipMemcpy(plan->tables, tb.bytes.data(), tb.bytes.size(), hipMemcpyHostToDevice));
+    }
+    *out = plan.release();
+    return RK_OK;
+}
+
+static int launch_pass(const PassPlan& pp, rk_pass_args& a, hipStream_t stream, const rk_user_blob* load_blob,
+                       const rk_user_blob* store_blob) {
+    // host loops over batch dims the kernel could not fold
+    const size_t nl = pp.loops.size();
+    int64_t total = 1;
+    for (auto& l : pp.loops) total *= l.n;
+    const rk_i64 in0 = a.in_off, out0 = a.out_off;
+    for (int64_t it = 0; it < total; ++it) {
+        int64_t rem = it;
+        a.in_off = in0;
+        a.out_off = out0;
+        for (size_t k = nl; k-- > 0;) {
+            const int64_t idx = rem % pp.loops[k].n;
+            rem /= pp.loops[k].n;
+            a.in_off += idx * pp.loops[k].in_str;
+            a.out_off += idx * pp.loops[k].out_str;
+        }
+        // one tile per work-group; grids beyond the launch limit are split
+        for (rk_u64 base = 0; base < a.n_tiles; base += pp.grid) {
+            a.tile_base = base;
+            const unsigned grid = (unsigned)std::min<rk_u64>(pp.grid, a.n_tiles - base);
+            // fused kernels take (pass args, load blob, store blob)
+            void* params[] = {&a, const_cast<rk_user_blob*>(load_blob), const_cast<rk_user_blob*>(store_blob)};
+            if (pp.jit) {
+                RK_HIP(hipModuleLaunchKernel(pp.jit, grid, 1, 1, (unsigned)pp.threads, 1, 1, 0, stream, params,
+                                             nullptr));
+            } else {
+                RK_HIP(hipLaunchKernel(pp.aot, dim3(grid), dim3((unsigned)pp.threads), params, 0, stream));
+            }
+        }
+    }
+    a.in_off = in0;
+    a.out_off = out0;
+    return RK_OK;
+}
+
+static int plan_execute(const rk_fft_plan* plan, void* out, const void* in, int inverse, void* ws, hipStream_t stream,
+                        const void* load_blob, size_t load_bytes, const void* store_blob, size_t store_bytes) {
+    if (!plan) return fail(RK_ERR_INVALID, "rk_fft_execute: null plan");
+    if (!out || !in) return fail(RK_ERR_INVALID, "rk_fft_execute: null in/out pointer");
+    if (plan->trivial) {
+        CopyArgs c = plan->copy;
+        c.in = (const float2*)in;
+        c.out = (float2*)out;
+        return launch_copy(c, stream);
+    }
+    if (plan->ws_bytes && !ws) return fail(RK_ERR_INVALID, "rk_fft_execute: plan needs %zu bytes of workspace", plan->ws_bytes);
+    void* bufs[4] = {const_cast<void*>(in), out, ws, ws ? (char*)ws + plan->ws1_offset : nullptr};
+    const char* tables = (const char*)plan->tables;
+    auto tab = [&](int64_t off) -> const void* { return off < 0 ? nullptr : tables + off; };
+    rk_user_blob lb, sb;
+    memset(&lb, 0, sizeof(lb));
+    memset(&sb, 0, sizeof(sb));
+    if (load_blob && load_bytes) memcpy(lb.b, load_blob, std::min<size_t>(load_bytes, sizeof(lb.b)));
+    if (store_blob && store_bytes) memcpy(sb.b, store_blob, std::min<size_t>(store_bytes, sizeof(sb.b)));
+    for (size_t p = 0; p < plan->passes.size(); ++p) {
+        const PassPlan& pp = plan->passes[p];
+        rk_pass_args a = pp.args;
+        a.in = (const float2*)bufs[pp.src];
+        a.out = (float2*)bufs[pp.dst];
+        a.twl = (const float2*)tab(pp.twl_off);
+        a.tw_hi = (const double2*)tab(pp.twhi_off);
+        a.tw_lo = (const double2*)tab(pp.twlo_off);
+        a.chirp = (const float2*)tab(pp.chirp_off);
+        a.bspec = (const float2*)tab(pp.bspec_off);
+        const bool norm_forward = (plan->desc.flags & RK_FFT_NORM_FORWARD) != 0;
+        if (inverse) {
+            if (pp.is_first) a.in_sign = -1.0f;
+            if (pp.is_last) {
+                a.out_sign = -1.0f;
+                if (!norm_forward) a.out_scale = (float)plan->norm;
+            }
+        } else if (norm_forward && pp.is_last) {
+            a.out_scale = (float)plan->norm;
+        }
+        if (plan->timing) RK_HIP(hipEventRecord(plan->events[2 * p], stream));
+        int rc = launch_pass(pp, a, stream, &lb, &sb);
+        if (rc != RK_OK) return rc;
+        if (plan->timing) RK_HIP(hipEventRecord(plan->events[2 * p + 1], stream));
+    }
+    return RK_OK;
+}
+
+}  // namespace rk
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+
+extern "C" int rk_fft_plan_create(const rk_fft_desc* desc, rk_fft_plan** plan) {
+    return rk::plan_create(desc, plan);
+}
+
+extern "C" int rk_fft_plan_workspace_size(const rk_fft_plan* plan, size_t* bytes) {
+    if (!plan || !bytes) return fail(RK_ERR_INVALID, "rk_fft_plan_workspace_size: null argument");
+    *bytes = plan->ws_bytes;
+    return RK_OK;
+}
+
+extern "C" int rk_fft_execute(const rk_fft_plan* plan, void* out, const void* in, int inverse, void* workspace,
+                              void* stream) {
+    return rk::plan_execute(plan, out, in, inverse, workspace, (hipStream_t)stream, nullptr, 0, nullptr, 0);
+}
+
+extern "C" int rk_fft_execute_fused(const rk_fft_plan* plan, void* out, const void* in, int inverse, void* workspace,
+                                    void* stream, const void* load_blob, size_t load_bytes, const void* store_blob,
+                                    size_t store_bytes) {
+    if (load_bytes > RK_FUSED_BLOB_BYTES || store_bytes > RK_FUSED_BLOB_BYTES)
+        return fail(RK_ERR_INVALID, "fused argument blob larger than %d bytes", RK_FUSED_BLOB_BYTES);
+    return rk::plan_execute(plan, out, in, inverse, workspace, (hipStream_t)stream, load_blob, load_bytes, store_blob,
+                            store_bytes);
+}
+
+extern "C" int rk_fft_plan_destroy(rk_fft_plan* plan) {
+    if (!plan) return RK_OK;
+    if (plan->tables) hipFree(plan->tables);
+    for (hipEvent_t e : plan->events) hipEventDestroy(e);
+    delete plan;
+    return RK_OK;
+}
+
+extern "C" int rk_fft_plan_num_passes(const rk_fft_plan* plan, int* n) {
+    if (!plan || !n) return fail(RK_ERR_INVALID, "rk_fft_plan_num_passes: null argument");
+    *n = plan->trivial ? 0 : (int)plan->passes.size();
+    return RK_OK;
+}
+
+extern "C" int rk_fft_plan_pass_info(const rk_fft_plan* plan, int pass, rk_fft_pass_info* info) {
+    if (!plan || !info) return fail(RK_ERR_INVALID, "rk_fft_plan_pass_info: null argument");
+    if (pass < 0 || pass >= (int)plan->passes.size()) return fail(RK_ERR_INVALID, "pass index %d out of range", pass);
+    const rk::PassPlan& pp = plan->passes[pass];
+    memset(info, 0, sizeof(*info));
+    info->L = pp.L;
+    info->C = pp.C;
+    info->threads = pp.threads;
+    info->kind = pp.kind;
+    info->load_mode = (int)pp.args.load_mode;
+    info->store_mode = (int)pp.args.store_mode;
+    info->src = pp.src;
+    info->dst = pp.dst;
+    info->axis = pp.axis;
+    info->pre_ops = pp.args.pre_ops;
+    info->post_ops = pp.args.post_ops;
+    info->n_inst = pp.args.n_inst;
+    info->grid = pp.grid;
+    info->axis_len = pp.args.N;
+    info->Ns = pp.args.dim_NS;
+    info->lds_bytes = pp.lds_bytes;
+    info->jit = pp.jit ? 1 : 0;
+    info->variant = pp.variant;
+    info->in_dims[0] = (int64_t)(pp.args.n_inst / (pp.args.dim_I * pp.args.dim_NS * pp.args.dim_H));
+    info->in_dims[1] = (int64_t)pp.args.n_valid_in;
+    info->in_dims[2] = (int64_t)pp.args.dim_I;
+    return RK_OK;
+}
+
+extern "C" int rk_fft_plan_set_pass_kernel(rk_fft_plan* plan, int pass, void* function) {
+    if (!plan || !function) return fail(RK_ERR_INVALID, "rk_fft_plan_set_pass_kernel: null argument");
+    if (pass < 0 || pass >= (int)plan->passes.size()) return fail(RK_ERR_INVALID, "pass index %d out of range", pass);
+    plan->passes[pass].jit = (hipFunction_t)function;
+    plan->passes[pass].aot = nullptr;
+    plan->passes[pass].user_kernel = true;
+    return RK_OK;
+}
+
+extern "C" int rk_fft_plan_set_timing(rk_fft_plan* plan, int enable) {
+    if (!plan) return fail(RK_ERR_INVALID, "rk_fft_plan_set_timing: null plan");
+    if (enable && plan->events.empty()) {
+        plan->events.resize(2 * plan->passes.size());
+        for (auto& e : plan->events) RK_HIP(hipEventCreate(&e));
+    }
+    plan->timing = enable != 0;
+    return RK_OK;
+}
+
+extern "C" int rk_fft_plan_pass_time_ms(const rk_fft_plan* plan, int pass, float* ms) {
+    if (!plan || !ms) return fail(RK_ERR_INVALID, "rk_fft_plan_pass_time_ms: null argument");
+    *ms = -1.0f;
+    if (!plan->timing || pass < 0 || pass >= (int)plan->passes.size()) return RK_OK;
+    RK_HIP(hipEventSynchronize(plan->events[2 * pass + 1]));
+    RK_HIP(hipEventElapsedTime(ms, plan->events[2 * pass], plan->events[2 * pass + 1]));
+    return RK_OK;
+}

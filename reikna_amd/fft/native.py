@@ -1,0 +1,147 @@
+"""Python handle of a native FFT plan (rk_fft_plan_create / rk_fft_execute)."""
+
+from __future__ import annotations
+
+import ctypes
+import weakref
+from collections.abc import Sequence
+from dataclasses import dataclass
+from typing import Any
+
+import numpy
+
+from .. import _lib
+from ..runtime import ArrayMetadata
+
+LOAD_MODES = {0: "direct", 1: "column", 2: "flat"}
+BUFFERS = {0: "in", 1: "out", 2: "ws0", 3: "ws1"}
+
+
+@dataclass
+class PassInfo:
+    L: int
+    C: int
+    threads: int
+    kind: int
+    load_mode: str
+    store_mode: str
+    src: str
+    dst: str
+    axis: int
+    pre_ops: int
+    post_ops: int
+    n_inst: int
+    grid: int
+    axis_len: int
+    Ns: int
+    lds_bytes: int
+    jit: bool
+    variant: int
+
+
+def _element_layout(meta: ArrayMetadata) -> tuple[list[int], int]:
+    item = meta.dtype.itemsize
+    if meta.first_element_offset % item or any(s % item for s in meta.strides):
+        raise ValueError("strides and offset must be multiples of the item size")
+    return [s // item for s in meta.strides], meta.first_element_offset // item
+
+
+class NativeFftPlan:
+    """A compiled pass chain for complex64 data of a given shape/layout/axes."""
+
+    def __init__(
+        self,
+        in_meta: ArrayMetadata,
+        out_meta: ArrayMetadata,
+        axes: Sequence[int] | None,
+        flags: int = 0,
+    ):
+        if in_meta.shape != out_meta.shape:
+            raise ValueError("input and output shapes differ")
+        if in_meta.dtype != numpy.complex64 or out_meta.dtype != numpy.complex64:
+            raise NotImplementedError(
+                f"the gfx950 FFT kernels compute in complex64; got {in_meta.dtype} -> {out_meta.dtype}"
+            )
+        ndim = len(in_meta.shape)
+        if ndim > _lib.RK_MAX_DIMS:
+            raise ValueError(f"at most {_lib.RK_MAX_DIMS} dimensions are supported")
+        desc = _lib.FftDesc()
+        desc.dtype = 0
+        desc.ndim = ndim
+        in_str, in_off = _element_layout(in_meta)
+        out_str, out_off = _element_layout(out_meta)
+        for k in range(ndim):
+            desc.shape[k] = in_meta.shape[k]
+            desc.in_strides[k] = in_str[k]
+            desc.out_strides[k] = out_str[k]
+        desc.in_offset = in_off
+        desc.out_offset = out_off
+        axes = list(range(ndim)) if axes is None else list(axes)
+        desc.naxes = len(axes)
+        for k, a in enumerate(axes):
+            desc.axes[k] = a
+        desc.flags = flags
+        self.shape = tuple(in_meta.shape)
+        self.axes = tuple(axes)
+        h = ctypes.c_void_p()
+        _lib.call("rk_fft_plan_create", ctypes.byref(desc), ctypes.byref(h))
+        self.handle = h.value
+        self._finalizer = weakref.finalize(self, _destroy, self.handle)
+        ws = ctypes.c_size_t(0)
+        _lib.call("rk_fft_plan_workspace_size", ctypes.c_void_p(self.handle), ctypes.byref(ws))
+        self.workspace_bytes = int(ws.value)
+
+    @property
+    def passes(self) -> list[PassInfo]:
+        n = ctypes.c_int(0)
+        _lib.call("rk_fft_plan_num_passes", ctypes.c_void_p(self.handle), ctypes.byref(n))
+        out = []
+        for p in range(n.value):
+            info = _lib.FftPassInfo()
+            _lib.call("rk_fft_plan_pass_info", ctypes.c_void_p(self.handle), p, ctypes.byref(info))
+            out.append(
+                PassInfo(
+                    L=info.L, C=info.C, threads=info.threads, kind=info.kind,
+                    load_mode=LOAD_MODES[info.load_mode], store_mode=LOAD_MODES[info.store_mode],
+                    src=BUFFERS[info.src], dst=BUFFERS[info.dst], axis=info.axis, pre_ops=info.pre_ops,
+                    post_ops=info.post_ops, n_inst=info.n_inst, grid=info.grid, axis_len=info.axis_len,
+                    Ns=info.Ns, lds_bytes=info.lds_bytes, jit=bool(info.jit), variant=info.variant,
+                )
+            )
+        return out
+
+    def execute(self, out_ptr: int, in_ptr: int, inverse: int, ws_ptr: int | None, stream: int | None,
+                load_blob: bytes | None = None, store_blob: bytes | None = None) -> None:
+        h = ctypes.c_void_p(self.handle)
+        if load_blob is None and store_blob is None:
+            rc = _lib.load().rk_fft_execute(h, ctypes.c_void_p(out_ptr), ctypes.c_void_p(in_ptr), int(inverse),
+                                            ctypes.c_void_p(ws_ptr), ctypes.c_void_p(stream))
+        else:
+            lb = load_blob or b""
+            sb = store_blob or b""
+            rc = _lib.load().rk_fft_execute_fused(
+                h, ctypes.c_void_p(out_ptr), ctypes.c_void_p(in_ptr), int(inverse), ctypes.c_void_p(ws_ptr),
+                ctypes.c_void_p(stream), lb, len(lb), sb, len(sb),
+            )
+        _lib.check(rc, "rk_fft_execute")
+
+    def set_pass_kernel(self, pass_index: int, function: int) -> None:
+        _lib.call("rk_fft_plan_set_pass_kernel", ctypes.c_void_p(self.handle), pass_index, ctypes.c_void_p(function))
+
+    def set_timing(self, enable: bool) -> None:
+        _lib.call("rk_fft_plan_set_timing", ctypes.c_void_p(self.handle), int(bool(enable)))
+
+    def pass_times_ms(self) -> list[float]:
+        out = []
+        for p in range(len(self.passes)):
+            ms = ctypes.c_float(-1)
+            _lib.call("rk_fft_plan_pass_time_ms", ctypes.c_void_p(self.handle), p, ctypes.byref(ms))
+            out.append(float(ms.value))
+        return out
+
+
+def _destroy(handle: Any) -> None:
+    try:
+        _lib.load().rk_fft_plan_destroy(ctypes.c_void_p(handle))
+    except Exception:  # pragma: no cover - interpreter shutdown
+        pass
