@@ -145,6 +145,20 @@ def main(argv=None) -> int:
     pass_ms = [sum(v) / len(v) for v in per_pass]
     dom = max(range(len(pass_ms)), key=lambda k: pass_ms[k])
 
+    # device-to-device copy bandwidth of the same array on this box (reference point)
+    for _ in range(3):
+        arr_out.copy_from(queue, arr_in)
+    queue.synchronize()
+    c0, c1 = Event(), Event()
+    c0.record(queue)
+    ncopy = 10
+    for _ in range(ncopy):
+        arr_out.copy_from(queue, arr_in)
+    c1.record(queue)
+    queue.synchronize()
+    copy_gbs = 2.0 * arr_in.nbytes * ncopy / (c0.elapsed_ms(c1) / 1e3) / 1e9
+    fftc(queue, arr_out, arr_in, 0)
+
     # correctness spot check against the FP64 oracle
     res = arr_out.get(queue)
     if 0 in axes:
@@ -191,6 +205,8 @@ def main(argv=None) -> int:
             "transform_GBs": round(transform_gbs, 1),
             "transform_frac": round(transform_gbs / HBM_PEAK_GBS, 4),
             "pass_us": [round(t * 1e3, 2) for t in pass_ms],
+            "d2d_copy_GBs": round(copy_gbs, 1),
+            "transform_frac_of_copy": round(transform_gbs / copy_gbs, 4),
         },
         "rel_l2": err,
         "rel_l2_bound": tol,

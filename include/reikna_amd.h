@@ -111,6 +111,9 @@ int rk_stream_wait_event(void* stream, void* event);
    keyed by source, options and target.  include_dir is added as -I. */
 int rk_module_compile(const char* source, const char* options,
                       void** module, char* log, size_t log_size);
+/* Compile only (no device needed): returns RK_OK if the source compiles for
+   gfx950, else RK_ERR_NO_KERNEL with the compiler log in `log`. */
+int rk_module_check(const char* source, const char* options, char* log, size_t log_size);
 int rk_module_get_function(void* module, const char* name, void** function);
 int rk_module_unload(void* module);
 /* Launch a run-time compiled kernel: args is a packed argument buffer laid out
@@ -146,6 +149,8 @@ typedef struct {
 #define RK_FFT_NO_JIT          0x4u     /* never run hiprtc for missing lengths */
 #define RK_FFT_NORM_FORWARD    0x8u     /* scale the forward result by 1/prod(N), not the inverse */
 #define RK_FFT_OUT_LAST_ONLY   0x10u    /* only the last pass touches `out`: earlier axes finish in workspace */
+#define RK_FFT_DRY_RUN         0x20u    /* plan only: no device tables, no run-time compilation;
+                                           the plan can be inspected but not executed */
 
 /* Build a plan: radix factorisation, pass chain, buffer routing, and all
    device tables (twiddles in FP64 rounded once to FP32, Bluestein chirp from

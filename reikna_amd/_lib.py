@@ -31,6 +31,7 @@ RK_FFT_FORCE_MULTIPASS = 0x2
 RK_FFT_NO_JIT = 0x4
 RK_FFT_NORM_FORWARD = 0x8
 RK_FFT_OUT_LAST_ONLY = 0x10
+RK_FFT_DRY_RUN = 0x20
 
 RK_BATCH_SCALE = 0x1
 
@@ -145,6 +146,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "rk_event_elapsed_ms": (_I, [_P, _P, ctypes.POINTER(ctypes.c_float)]),
     "rk_stream_wait_event": (_I, [_P, _P]),
     "rk_module_compile": (_I, [ctypes.c_char_p, ctypes.c_char_p, _PP, ctypes.c_char_p, _SZ]),
+    "rk_module_check": (_I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, _SZ]),
     "rk_module_get_function": (_I, [_P, ctypes.c_char_p, _PP]),
     "rk_module_unload": (_I, [_P]),
     "rk_launch": (

@@ -305,8 +305,9 @@ class _Static:
 _module_cache: dict[str, tuple[int, int]] = {}
 
 
-def compile_kernel(source: str, name: str, options: Sequence[str] = ()) -> int:
-    """Compile HIP source with hiprtc (cached) and return the function handle."""
+def compile_kernel(source: str, name: str, options: Sequence[str] = (), *, check_only: bool = False) -> int:
+    """Compile HIP source with hiprtc (cached) and return the function handle
+    (with ``check_only``: compile without a device and return 0)."""
     key = hashlib.sha256((source + "\0" + " ".join(options) + "\0" + name).encode()).hexdigest()
     if key in _module_cache:
         return _module_cache[key][1]
@@ -314,6 +315,11 @@ def compile_kernel(source: str, name: str, options: Sequence[str] = ()) -> int:
     mod = ctypes.c_void_p()
     log = ctypes.create_string_buffer(65536)
     opts = " ".join(["-Xclang -target-feature -Xclang -packed-fp32-ops", *options])
+    if check_only:
+        rc = lib.rk_module_check(source.encode(), opts.encode(), log, len(log))
+        if rc != 0:
+            raise _lib.RkError(rc, f"kernel {name} failed to compile: {_lib.last_error()}\n--- source ---\n{source}")
+        return 0
     rc = lib.rk_module_compile(source.encode(), opts.encode(), ctypes.byref(mod), log, len(log))
     if rc != 0:
         raise _lib.RkError(rc, f"kernel {name} failed to compile: {_lib.last_error()}\n--- source ---\n{source}")
@@ -423,6 +429,11 @@ class ComputationPlan:
     def device(self) -> Device:
         return self._device
 
+    @property
+    def virtual(self) -> bool:
+        """True when planning against a VirtualDevice (no GPU: nothing is allocated)."""
+        return bool(getattr(self._device, "virtual", False))
+
     # -- arrays ------------------------------------------------------------
     def _scalar(self, val: Any) -> KernelArgument:
         name = self._translator(self._persistent_idgen())
@@ -436,7 +447,7 @@ class ComputationPlan:
         arr = numpy.ascontiguousarray(arr)
         ann = Annotation(ArrayMetadata.from_arraylike(arr), "i")
         self._internal_annotations[name] = ann
-        self._persistent_values[name] = Array.from_host(self._device, arr)
+        self._persistent_values[name] = arr if self.virtual else Array.from_host(self._device, arr)
         return KernelArgument(name, ann)
 
     def constant_array(self, arr: numpy.ndarray) -> KernelArgument:
@@ -444,7 +455,7 @@ class ComputationPlan:
         arr = numpy.ascontiguousarray(arr)
         ann = Annotation(ArrayMetadata.from_arraylike(arr), constant=True)
         self._internal_annotations[name] = ann
-        self._persistent_values[name] = Array.from_host(self._device, arr)
+        self._persistent_values[name] = arr if self.virtual else Array.from_host(self._device, arr)
         return KernelArgument(name, ann)
 
     def temp_array(
@@ -517,7 +528,7 @@ class ComputationPlan:
         body = render_with_modules(lambda: template_def.render(decl, *argobjects, **kwds))
         source = PRELUDE + sizes.source + body
         opts = list(self._compiler_options) + (["-ffast-math"] if self._fast_math else [])
-        fn = compile_kernel(source, kernel_name, opts)
+        fn = compile_kernel(source, kernel_name, opts, check_only=self.virtual)
         leaf_params = {p.name: p for p in subtree.get_leaf_parameters()}
         anns = [leaf_params[n].annotation for n in leaf_names]
         self._steps.append(PlannedKernelCall(fn, leaf_names, anns, sizes, adhoc, source, kernel_name))
@@ -574,6 +585,8 @@ class ComputationPlan:
                         deps.add_edge(n, other)
         internal: dict[str, Any] = dict(self._persistent_values)
         temps = []
+        if self.virtual:
+            return ComputationCallable(self._device, self._tr_tree.get_leaf_parameters(), self._steps, internal, temps)
         # temporaries whose live ranges do not overlap share one allocation
         pool: list[tuple[Array, set[str]]] = []
         for name in sorted(self._temp_arrays, key=lambda n: -self._internal_annotations[n].nbytes):

@@ -22,10 +22,14 @@ typedef unsigned long long rk_u64;
 typedef long long rk_i64;
 typedef unsigned int rk_u32;
 
+// Compute mapping of threads to (instance c, thread-in-instance t):
+//   row mapping    (cmap = 0): c = tid / T, t = tid % T  (lanes along an instance)
+//   column mapping (cmap = 1): c = tid % C, t = tid / C  (lanes across instances)
 // load/store modes: how global memory is walked
-#define RK_MODE_DIRECT 0u   // each thread touches its own instance's elements
-#define RK_MODE_COLUMN 1u   // lanes run across adjacent instances (unit instance stride)
-#define RK_MODE_FLAT 2u     // the tile is one contiguous block of C*L elements
+#define RK_MODE_DIRECT 0u   // each thread touches its own elements in the compute mapping
+#define RK_MODE_COLUMN 1u   // staged through LDS, lanes across adjacent instances
+#define RK_MODE_FLAT 2u     // staged through LDS, the tile is one contiguous block of C*L elements
+#define RK_MODE_ROW 3u      // staged through LDS, lanes along each instance's elements
 
 // pre-ops, applied to each loaded element (in this order)
 #define RK_PRE_CHIRP 0x1u     // Bluestein: x[m] * chirp[m], zero for m >= n_valid_in
@@ -57,7 +61,7 @@ struct rk_pass_args {
     rk_u32 tw_shift;
     rk_u32 load_mode, store_mode;
     rk_u32 pre_ops, post_ops;
-    rk_u32 _pad0;
+    rk_u32 cmap;             // 0 row mapping, 1 column mapping
     rk_u64 n_valid_in;       // positions m >= n_valid_in read as zero
     rk_u64 n_valid_out;      // positions kabs >= n_valid_out are not stored
     float in_sign;           // -1 conjugates the input (inverse transform)

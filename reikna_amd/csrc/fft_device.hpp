@@ -285,9 +285,17 @@ __device__ __forceinline__ Inst decompose(const rk_pass_args& a, rk_u64 inst) {
     r.valid = inst < a.n_inst;
     rk_u64 x = r.valid ? inst : 0;
     rk_u64 i = 0, l = 0, h = 0;
-    if (a.dim_I > 1) { i = x % a.dim_I; x /= a.dim_I; }
-    if (a.dim_NS > 1) { l = x % a.dim_NS; x /= a.dim_NS; }
-    if (a.dim_H > 1) { h = x % a.dim_H; x /= a.dim_H; }
+    if (a.n_inst <= 0xffffffffull) {   // 32-bit division is several times cheaper
+        rk_u32 y = (rk_u32)x;
+        if (a.dim_I > 1) { i = y % (rk_u32)a.dim_I; y /= (rk_u32)a.dim_I; }
+        if (a.dim_NS > 1) { l = y % (rk_u32)a.dim_NS; y /= (rk_u32)a.dim_NS; }
+        if (a.dim_H > 1) { h = y % (rk_u32)a.dim_H; y /= (rk_u32)a.dim_H; }
+        x = y;
+    } else {
+        if (a.dim_I > 1) { i = x % a.dim_I; x /= a.dim_I; }
+        if (a.dim_NS > 1) { l = x % a.dim_NS; x /= a.dim_NS; }
+        if (a.dim_H > 1) { h = x % a.dim_H; x /= a.dim_H; }
+    }
     r.o = x;
     r.i = i;
     r.l = l;
@@ -300,18 +308,41 @@ __device__ __forceinline__ Inst decompose(const rk_pass_args& a, rk_u64 inst) {
 
 // W_N^e for 0 <= e < N from two FP64 tables; the product is formed in FP64
 // and rounded to float once.
-__device__ __forceinline__ float2 twiddle_n(const rk_pass_args& a, rk_u64 e) {
-    double2 h = a.tw_hi[e >> a.tw_shift];
-    double2 l = a.tw_lo[e & a.tw_mask];
-    return make_float2((float)(h.x * l.x - h.y * l.y), (float)(h.x * l.y + h.y * l.x));
+__device__ __forceinline__ double2 d_mul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+
+__device__ __forceinline__ double2 twiddle_n64(const rk_pass_args& a, rk_u64 e) {
+    return d_mul(a.tw_hi[e >> a.tw_shift], a.tw_lo[e & a.tw_mask]);
+}
+
+// The inter-pass twiddles of one thread, W_N^{p*l*tw_mult} for p = p0, p0+dp,
+// p0+2dp, ...: two table lookups for the first one and the step, then an FP64
+// recurrence; every value is rounded to float once.
+struct TwiddleWalk {
+    double2 w, s;
+    __device__ __forceinline__ void init(const rk_pass_args& a, rk_u64 l, rk_u64 p0, rk_u64 dp) {
+        const rk_u64 lm = l * a.tw_mult;
+        w = twiddle_n64(a, p0 * lm);
+        s = twiddle_n64(a, dp * lm);
+    }
+    __device__ __forceinline__ float2 next() {
+        const float2 r = make_float2((float)w.x, (float)w.y);
+        w = d_mul(w, s);
+        return r;
+    }
+};
 
 // Plain strided element access: the identity IO policy.
 // An IO policy is an object with load(a, o, m, i, addr) and
 // store(a, o, k, i, addr, v): o / i are the flat indices over the dims before /
 // after the transformed axis, m / k the axis position, addr the plain strided
 // element address.  Fused transformations (generated at run time) use o, m, i.
+// kBranchFree: loads of elements outside the data may be redirected to element
+// 0 of the buffer and discarded, so every load of a thread is issued without a
+// branch around it (a branch per load serialises them on their completion).
 struct PlainIO {
+    static constexpr bool kBranchFree = true;
     __device__ __forceinline__ float2 load(const rk_pass_args& a, rk_u64 /*o*/, rk_u64 /*m*/, rk_u64 /*i*/,
                                            rk_i64 addr) const {
         return a.in[addr];
@@ -342,14 +373,17 @@ template <int L, int V, class IO>
 __device__ __forceinline__ float2 load_elem(const rk_pass_args& a, const IO& io, const Inst& ic, int p) {
     constexpr int PRE = Variant<V>::PRE;
     const rk_u64 m = ic.jpass + (rk_u64)p * a.pos_mult;
+    const bool ok = ic.valid && m < a.n_valid_in;
     float2 x = make_float2(0.0f, 0.0f);
-    if (ic.valid && m < a.n_valid_in) {
-        x = io.load(a, ic.o, m, ic.i, ic.in_base + (rk_i64)m * a.in_sa);
-        x.y *= a.in_sign;
-        if constexpr (PRE == RK_VAR_PRE_CHIRP) x = c_mul(x, a.chirp[m]);
-        if constexpr (PRE == RK_VAR_PRE_TWIDDLE) x = c_mul(x, twiddle_n(a, (rk_u64)p * ic.l * a.tw_mult));
+    if constexpr (IO::kBranchFree) {
+        const float2 y = io.load(a, ic.o, m, ic.i, ok ? ic.in_base + (rk_i64)m * a.in_sa : 0);
+        x = ok ? y : x;
+    } else {
+        if (ok) x = io.load(a, ic.o, m, ic.i, ic.in_base + (rk_i64)m * a.in_sa);
     }
-    return x;
+    x.y *= a.in_sign;
+    if constexpr (PRE == RK_VAR_PRE_CHIRP) x = c_mul(x, a.chirp[ok ? m : 0]);
+    return x;   // the inter-pass twiddle (PRE_TWIDDLE) is applied by the caller's TwiddleWalk
 }
 
 template <int L, int V, class IO>
@@ -365,26 +399,106 @@ __device__ __forceinline__ void store_elem(const rk_pass_args& a, const IO& io, 
 }
 
 // ------------------------------------------------------------------------
+// LDS exchange between two register layouts
+// ------------------------------------------------------------------------
+// Every value a thread holds in `w` is written to LDS at wpos(i) (skipped
+// when negative) and every value of `r` is read from rpos(i).  With SPLIT the
+// real and imaginary parts travel separately through a buffer of floats, so
+// the tile needs half the LDS (two work-groups then fit on a CU); `w` and `r`
+// may be the same array.
+template <bool SPLIT, int NW, int NR, class FW, class FR>
+__device__ __forceinline__ void lds_exchange(void* base, const float2* w, float2* r, FW wpos, FR rpos) {
+    if constexpr (!SPLIT) {
+        float2* lds = reinterpret_cast<float2*>(base);
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+            const int p = wpos(i);
+            if (p >= 0) lds[p] = w[i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            const int p = rpos(i);
+            if (p >= 0) r[i] = lds[p];
+        }
+    } else {
+        float* f = reinterpret_cast<float*>(base);
+        float wy[NW], rx[NR];
+#pragma unroll
+        for (int i = 0; i < NW; ++i) wy[i] = w[i].y;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+            const int p = wpos(i);
+            if (p >= 0) f[p] = w[i].x;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            const int p = rpos(i);
+            rx[i] = p >= 0 ? f[p] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+            const int p = wpos(i);
+            if (p >= 0) f[p] = wy[i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            const int p = rpos(i);
+            if (p >= 0) r[i] = make_float2(rx[i], f[p]);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
 // the Stockham stages of one tile
 // ------------------------------------------------------------------------
-template <int L, int S, int VM>
-__device__ __forceinline__ void run_stage(float2 (&v)[VM], float2* my, int t, const float2* twl,
-                                          bool from_lds) {
+// Stage s holds, in thread t, the butterflies j = t + q*T (q < Q_s, j < NB_s);
+// v[q*R + r] is input r of butterfly j, at position j + r*NB_s of the
+// instance.  The butterfly's outputs go to (j/Ns)*Ns*R + j%Ns + r*Ns, which is
+// where stage s+1 reads them; after the last stage v[q*R + r] holds output
+// k = j + r*NB.
+template <int L, int S>
+__device__ __forceinline__ bool stage_valid(int t, int q) {
+    using P = Plan<L>;
+    return (P::nb(S) % P::T) == 0 || (t + q * P::T) < P::nb(S);
+}
+
+template <int L, int S>
+__device__ __forceinline__ int stage_in_pos(int t, int i) {   // input position of v[i]
+    using P = Plan<L>;
+    constexpr int R = P::radix(S);
+    const int q = i / R, r = i % R;
+    return (t + q * P::T) + r * P::nb(S);
+}
+
+template <int L, int S>
+__device__ __forceinline__ int stage_out_pos(int t, int i) {  // output position of v[i]
     using P = Plan<L>;
     constexpr int R = P::radix(S);
     constexpr int NS = P::ns(S);
-    constexpr int NB = P::nb(S);
+    const int q = i / R, r = i % R;
+    const int j = t + q * P::T;
+    return (j / NS) * NS * R + (j % NS) + r * NS;
+}
+
+template <int L, int S, bool SPLIT, int VM>
+__device__ __forceinline__ void run_stages(float2 (&v)[VM], void* my, int t, const float2* twl) {
+    using P = Plan<L>;
+    constexpr int R = P::radix(S);
+    constexpr int NS = P::ns(S);
     constexpr int Q = P::q(S);
     constexpr int T = P::T;
-    constexpr bool EVEN = (NB % T) == 0;
-
     if constexpr (S > 0) {
+        // stage twiddles W_{NS*R}^{r*(j%NS)} from the FP64-rounded W_L table
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const int j = t + q * T;
-            if (EVEN || j < NB) {
-#pragma unroll
-                for (int r = 0; r < R; ++r) v[q * R + r] = my[lds_pad(j + r * NB)];
+            if (stage_valid<L, S>(t, q)) {
                 const int m = j % NS;
 #pragma unroll
                 for (int r = 1; r < R; ++r)
@@ -393,44 +507,30 @@ __device__ __forceinline__ void run_stage(float2 (&v)[VM], float2* my, int t, co
         }
     }
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const int j = t + q * T;
-        if (EVEN || j < NB) dft<R>(&v[q * R]);
-    }
+    for (int q = 0; q < Q; ++q)
+        if (stage_valid<L, S>(t, q)) dft<R>(&v[q * R]);
     if constexpr (S + 1 < P::NSTAGE) {
-        if (S > 0 || from_lds) __syncthreads();   // every read of the tile is done
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            const int j = t + q * T;
-            if (EVEN || j < NB) {
-                const int base = (j / NS) * NS * R + (j % NS);
-#pragma unroll
-                for (int r = 0; r < R; ++r) my[lds_pad(base + r * NS)] = v[q * R + r];
-            }
-        }
-        __syncthreads();
-        run_stage<L, S + 1, VM>(v, my, t, twl, true);
+        constexpr int R1 = P::radix(S + 1);
+        constexpr int Q1 = P::q(S + 1);
+        lds_exchange<SPLIT, Q * R, Q1 * R1>(
+            my, v, v,
+            [&](int i) { return stage_valid<L, S>(t, i / R) ? lds_pad(stage_out_pos<L, S>(t, i)) : -1; },
+            [&](int i) { return stage_valid<L, S + 1>(t, i / R1) ? lds_pad(stage_in_pos<L, S + 1>(t, i)) : -1; });
+        run_stages<L, S + 1, SPLIT, VM>(v, my, t, twl);
     }
-}
-
-// Position of the value held in v[q*R + r] after the last stage.
-template <int L>
-__device__ __forceinline__ int out_pos(int t, int q, int r) {
-    using P = Plan<L>;
-    constexpr int S = P::NSTAGE - 1;
-    return t + q * P::T + r * P::nb(S);
-}
-
-template <int L>
-__device__ __forceinline__ bool out_valid(int t, int q) {
-    using P = Plan<L>;
-    constexpr int S = P::NSTAGE - 1;
-    return (P::nb(S) % P::T) == 0 || (t + q * P::T) < P::nb(S);
 }
 
 // ------------------------------------------------------------------------
 // the pass kernel body
 // ------------------------------------------------------------------------
+// LDS bytes of a tile and whether its exchanges are split into halves.
+template <int L, int C> __host__ __device__ constexpr bool tile_split() {
+    return C * lds_pitch<L>() * 8 > 80 * 1024;
+}
+template <int L, int C> __host__ __device__ constexpr int tile_lds_bytes() {
+    return C * lds_pitch<L>() * (tile_split<L, C>() ? 4 : 8);
+}
+
 template <int L, int C, int V, class IO>
 __device__ __forceinline__ void fft_pass_body(const rk_pass_args& a, const IO& io) {
     using P = Plan<L>;
@@ -443,125 +543,126 @@ __device__ __forceinline__ void fft_pass_body(const rk_pass_args& a, const IO& i
     constexpr int RL = P::radix(SL);
     constexpr int QL = P::q(SL);
     constexpr bool BLUE = Variant<V>::BLUE;
-    __shared__ float2 lds[C * LP];
+    constexpr bool TWID = Variant<V>::PRE == RK_VAR_PRE_TWIDDLE;
+    constexpr bool SPLIT = tile_split<L, C>();
+    __shared__ float2 lds[SPLIT ? (C * LP + 1) / 2 : C * LP];
+
+    const rk_u64 tile = (rk_u64)blockIdx.x + a.tile_base;
+    if (tile >= a.n_tiles) return;
+    const rk_u64 inst0 = tile * (rk_u64)C;
 
     const int tid = threadIdx.x;
-    const int c = tid / T;
-    const int t = tid - c * T;
-    float2* my = lds + c * LP;
-    // column mapping: lanes across instances
-    const int cc = tid % C;
-    const int pc = tid / C;
+    // compute mapping (see fft_args.h): lanes along an instance or across instances
+    const bool colmap = a.cmap != 0;
+    const int c = colmap ? (tid % C) : (tid / T);
+    const int t = colmap ? (tid / C) : (tid % T);
+    const int cbase = c * LP;
+    // the other mapping, for a side staged through LDS
+    const int cs = colmap ? (tid / T) : (tid % C);
+    const int ps = colmap ? (tid % T) : (tid / C);
 
-    // One tile per work-group: a loop over tiles would let the compiler keep
-    // every per-thread address of the stages live across iterations.
-    {
-        const rk_u64 tile = (rk_u64)blockIdx.x + a.tile_base;
-        if (tile >= a.n_tiles) return;
-        const rk_u64 inst0 = tile * (rk_u64)C;
-        const Inst ic = decompose<L>(a, inst0 + c);
-        float2 v[VM];
+    float2 v[VM];
+    float2 g[R0];   // staging registers (global side of a staged load/store)
 
-        // ---- stage-0 inputs: element p = t + r*T ----
-        bool from_lds = false;
-        if (a.load_mode == RK_MODE_DIRECT) {
+    // ---- stage-0 inputs: element p = t + r*T of instance c ----
+    const Inst ic = decompose<L>(a, inst0 + c);
+    auto own_pos = [&](int i) { return cbase + lds_pad(stage_in_pos<L, 0>(t, i)); };
+    if (a.load_mode == RK_MODE_DIRECT) {
 #pragma unroll
-            for (int r = 0; r < R0; ++r) v[r] = load_elem<L, V, IO>(a, io, ic, t + r * T);
-        } else {
-            if (a.load_mode == RK_MODE_COLUMN) {
-                const Inst icc = decompose<L>(a, inst0 + cc);
+        for (int r = 0; r < R0; ++r) v[r] = load_elem<L, V, IO>(a, io, ic, t + r * T);
+        if constexpr (TWID) {
+            TwiddleWalk tw;
+            tw.init(a, ic.l, t, T);
 #pragma unroll
-                for (int it = 0; it < R0; ++it) {
-                    const int p = pc + it * T;
-                    lds[cc * LP + lds_pad(p)] = load_elem<L, V, IO>(a, io, icc, p);
-                }
-            } else {   // FLAT: instance-linear block of C*L elements, no pre-ops
-                const rk_i64 base = a.in_off + (rk_i64)inst0 * L;
+            for (int r = 0; r < R0; ++r) v[r] = c_mul(v[r], tw.next());
+        }
+    } else if (a.load_mode == RK_MODE_FLAT) {
+        // instance-linear block of C*L elements, no pre-ops
+        const rk_i64 base = a.in_off + (rk_i64)inst0 * L;
 #pragma unroll
-                for (int it = 0; it < R0; ++it) {
-                    const int e = tid + it * NT;
-                    const int ce = e / L, p = e - ce * L;
-                    float2 x = make_float2(0.0f, 0.0f);
-                    if (inst0 + ce < a.n_inst) {
-                        x = io.load(a, inst0 + ce, p, 0, base + e);
-                        x.y *= a.in_sign;
-                    }
-                    lds[ce * LP + lds_pad(p)] = x;
-                }
+        for (int it = 0; it < R0; ++it) {
+            const int e = tid + it * NT;
+            const int ce = e / L, p = e - ce * L;
+            float2 x = make_float2(0.0f, 0.0f);
+            const bool ok = inst0 + ce < a.n_inst;
+            if constexpr (IO::kBranchFree) {
+                const float2 y = io.load(a, inst0 + ce, p, 0, ok ? base + e : 0);
+                x = ok ? y : x;
+            } else {
+                if (ok) x = io.load(a, inst0 + ce, p, 0, base + e);
             }
-            __syncthreads();
-#pragma unroll
-            for (int r = 0; r < R0; ++r) v[r] = my[lds_pad(t + r * T)];
-            from_lds = true;
+            x.y *= a.in_sign;
+            g[it] = x;
         }
-
-        run_stage<L, 0, VM>(v, my, t, a.twl, from_lds);
-
-        if constexpr (BLUE) {
-            // A = FFT(chirp * x): multiply by the kernel spectrum, conjugate,
-            // and transform again (the inverse DFT as conj(FFT(conj .))).
+        lds_exchange<SPLIT, R0, R0>(
+            lds, g, v,
+            [&](int it) { const int e = tid + it * NT; const int ce = e / L; return ce * LP + lds_pad(e - ce * L); },
+            own_pos);
+    } else {
+        // COLUMN (under the row mapping) or ROW (under the column mapping)
+        const Inst ics = decompose<L>(a, inst0 + cs);
+        TwiddleWalk tw;
+        if constexpr (TWID) tw.init(a, ics.l, ps, T);
 #pragma unroll
-            for (int q = 0; q < QL; ++q)
-                if (out_valid<L>(t, q)) {
-#pragma unroll
-                    for (int r = 0; r < RL; ++r) {
-                        const int k = out_pos<L>(t, q, r);
-                        v[q * RL + r] = c_conj(c_mul(v[q * RL + r], a.bspec[k]));
-                    }
-                }
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < QL; ++q)
-                if (out_valid<L>(t, q)) {
-#pragma unroll
-                    for (int r = 0; r < RL; ++r) my[lds_pad(out_pos<L>(t, q, r))] = v[q * RL + r];
-                }
-            __syncthreads();
-#pragma unroll
-            for (int r = 0; r < R0; ++r) v[r] = my[lds_pad(t + r * T)];
-            run_stage<L, 0, VM>(v, my, t, a.twl, true);
+        for (int it = 0; it < R0; ++it) {
+            float2 x = load_elem<L, V, IO>(a, io, ics, ps + it * T);
+            if constexpr (TWID) x = c_mul(x, tw.next());
+            g[it] = x;
         }
+        lds_exchange<SPLIT, R0, R0>(
+            lds, g, v, [&](int it) { return cs * LP + lds_pad(ps + it * T); }, own_pos);
+    }
 
-        // ---- store: output element k ----
-        if (a.store_mode == RK_MODE_DIRECT) {
+    // LDS of this thread's instance (the stages index it relative to the instance)
+    void* my = SPLIT ? (void*)(reinterpret_cast<float*>(lds) + cbase) : (void*)(lds + cbase);
+    auto last_out = [&](int i) {
+        return stage_valid<L, SL>(t, i / RL) ? cbase + lds_pad(stage_out_pos<L, SL>(t, i)) : -1;
+    };
+    run_stages<L, 0, SPLIT, VM>(v, my, t, a.twl);
+
+    if constexpr (BLUE) {
+        // A = FFT(chirp * x): multiply by the kernel spectrum, conjugate, and
+        // transform again (the inverse DFT as conj(FFT(conj .))).
 #pragma unroll
-            for (int q = 0; q < QL; ++q)
-                if (out_valid<L>(t, q)) {
+        for (int i = 0; i < QL * RL; ++i)
+            if (stage_valid<L, SL>(t, i / RL)) {
+                const int k = stage_out_pos<L, SL>(t, i);
+                v[i] = c_conj(c_mul(v[i], a.bspec[k]));
+            }
+        lds_exchange<SPLIT, QL * RL, R0>(lds, v, v, last_out, own_pos);
+        run_stages<L, 0, SPLIT, VM>(v, my, t, a.twl);
+    }
+
+    // ---- store: output element k = stage_out_pos of the last stage ----
+    if (a.store_mode == RK_MODE_DIRECT) {
 #pragma unroll
-                    for (int r = 0; r < RL; ++r)
-                        store_elem<L, V, IO>(a, io, ic, out_pos<L>(t, q, r), v[q * RL + r]);
-                }
-        } else {
-            __syncthreads();
+        for (int i = 0; i < QL * RL; ++i)
+            if (stage_valid<L, SL>(t, i / RL)) store_elem<L, V, IO>(a, io, ic, stage_out_pos<L, SL>(t, i), v[i]);
+    } else if (a.store_mode == RK_MODE_FLAT) {
+        // no post-ops beyond sign and scale
+        lds_exchange<SPLIT, QL * RL, R0>(lds, v, g, last_out, [&](int it) {
+            const int e = tid + it * NT;
+            const int ce = e / L;
+            return ce * LP + lds_pad(e - ce * L);
+        });
+        const rk_i64 base = a.out_off + (rk_i64)inst0 * L;
 #pragma unroll
-            for (int q = 0; q < QL; ++q)
-                if (out_valid<L>(t, q)) {
-#pragma unroll
-                    for (int r = 0; r < RL; ++r) my[lds_pad(out_pos<L>(t, q, r))] = v[q * RL + r];
-                }
-            __syncthreads();
-            if (a.store_mode == RK_MODE_COLUMN) {
-                const Inst icc = decompose<L>(a, inst0 + cc);
-#pragma unroll
-                for (int it = 0; it < R0; ++it) {
-                    const int k = pc + it * T;
-                    store_elem<L, V, IO>(a, io, icc, k, lds[cc * LP + lds_pad(k)]);
-                }
-            } else {   // FLAT, no post-ops beyond sign and scale
-                const rk_i64 base = a.out_off + (rk_i64)inst0 * L;
-#pragma unroll
-                for (int it = 0; it < R0; ++it) {
-                    const int e = tid + it * NT;
-                    const int ce = e / L, k = e - ce * L;
-                    if (inst0 + ce < a.n_inst) {
-                        float2 y = lds[ce * LP + lds_pad(k)];
-                        y.y *= a.out_sign;
-                        y = c_scale(y, a.out_scale);
-                        io.store(a, inst0 + ce, k, 0, base + e, y);
-                    }
-                }
+        for (int it = 0; it < R0; ++it) {
+            const int e = tid + it * NT;
+            const int ce = e / L, k = e - ce * L;
+            if (inst0 + ce < a.n_inst) {
+                float2 y = g[it];
+                y.y *= a.out_sign;
+                y = c_scale(y, a.out_scale);
+                io.store(a, inst0 + ce, k, 0, base + e, y);
             }
         }
+    } else {
+        lds_exchange<SPLIT, QL * RL, R0>(lds, v, g, last_out,
+                                         [&](int it) { return cs * LP + lds_pad(ps + it * T); });
+        const Inst ics = decompose<L>(a, inst0 + cs);
+#pragma unroll
+        for (int it = 0; it < R0; ++it) store_elem<L, V, IO>(a, io, ics, ps + it * T, g[it]);
     }
 }
 

@@ -66,6 +66,13 @@ int tile_c_col(int L) {
     int T = threads_per_instance(L);
     int C = std::max(16, (tile_c_row(L) / 16) * 16);
     while (C > 1 && ((int64_t)C * T > 1024 || (int64_t)C * lds_pitch_host(L) * 8 > 159 * 1024)) C /= 2;
+    // tuning override: an upper bound on the column tile width (kernels outside
+    // the catalog come from the run-time compiler)
+    static const int cap = [] {
+        const char* e = getenv("RK_COL_TILE_MAX");
+        return e && *e ? atoi(e) : 0;
+    }();
+    if (cap > 0 && C > cap) C = cap;
     return C;
 }
 
@@ -156,6 +163,8 @@ struct PassPlan {
     int kind = 0;
     int64_t twl_off = -1, twhi_off = -1, twlo_off = -1, chirp_off = -1, bspec_off = -1;
     std::vector<HostLoop> loops;
+    uint64_t inst_per_outer = 0;   // H * NS * I: instances per outer-batch index
+    int64_t outer_count = 1;       // O
 };
 
 struct TableBuilder {
@@ -185,6 +194,13 @@ struct rk_fft_plan {
     bool timing = false;
     std::vector<hipEvent_t> events;   // 2 per pass when timing
     int device = 0;
+    // Chunked execution of a multi-pass single-axis plan: every pass runs on
+    // `chunk_outer` outer-batch indices before the next chunk starts, and the
+    // workspace holds one chunk, so a chunk's intermediate can stay in the
+    // Infinity Cache between the passes.  0 = off.
+    int64_t chunk_outer = 0;
+    int64_t n_outer = 1;
+    int64_t ws_outer_stride = 0;   // elements of one outer index in the workspace layout
 };
 
 namespace rk {
@@ -375,6 +391,12 @@ static int find_kernel(PassPlan& pp, uint32_t flags) {
     if (flags & RK_FFT_NO_JIT)
         return fail(RK_ERR_NO_KERNEL, "no precompiled kernel for L=%d C=%d variant=%d and run-time compilation is off",
                     pp.L, pp.C, pp.variant);
+    if (flags & RK_FFT_DRY_RUN) {   // describe the kernel the run-time compiler would build
+        pp.threads = pp.C * threads_per_instance(pp.L);
+        const int full = pp.C * lds_pitch_host(pp.L) * 8;
+        pp.lds_bytes = full > 80 * 1024 ? full / 2 : full;
+        return RK_OK;
+    }
     int rc = jit_pass_kernel(pp.L, pp.C, pp.variant, &pp.jit, &pp.threads, &pp.lds_bytes);
     if (rc != RK_OK) return rc;
     return RK_OK;
@@ -597,21 +619,32 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
             const int64_t inst_in = I > 1 ? si_in : (ap.NS > 1 || ap.H > 1) ? a.in_sa : so_in;
             const bool flat_in = ap.H == 1 && ap.NS == 1 && I == 1 && a.in_sa == 1 && (O == 1 || so_in == L) &&
                                  ap.pre_ops == 0 && ap.n_valid_in == ap.Nax && pp.loops.empty();
-            if (flat_in) a.load_mode = RK_MODE_FLAT;
-            else if (pos_in == 1) a.load_mode = RK_MODE_DIRECT;
-            else if (inst_in == 1) a.load_mode = RK_MODE_COLUMN;
-            else a.load_mode = RK_MODE_DIRECT;
             const int64_t pos_out = ap.NS * a.out_sa;
             const int64_t inst_out = I > 1 ? si_out : ap.NS > 1 ? a.out_sa : ap.H > 1 ? L * a.out_sa : so_out;
             const bool flat_out = ap.NS == 1 && I == 1 && a.out_sa == 1 &&
                                   (ap.H == 1 ? (O == 1 || so_out == L) : (O == 1 || so_out == ap.Nax)) &&
                                   ap.post_ops == 0 && ap.n_valid_out == ap.Nax && pp.loops.empty();
-            if (flat_out) a.store_mode = RK_MODE_FLAT;
-            else if (pos_out == 1) a.store_mode = RK_MODE_DIRECT;
-            else if (inst_out == 1) a.store_mode = RK_MODE_COLUMN;
-            else a.store_mode = RK_MODE_DIRECT;
-            const bool column = a.load_mode == RK_MODE_COLUMN || a.store_mode == RK_MODE_COLUMN;
-            pp.C = column ? tile_c_col(ap.L) : tile_c_row(ap.L);
+            // A side is column-like when adjacent instances are adjacent in
+            // memory and its elements are not; the compute mapping follows
+            // the column-like side(s) so they are read/written straight from
+            // registers, and a row-like side under that mapping is staged
+            // through LDS.
+            const bool col_in = pos_in != 1 && inst_in == 1;
+            const bool col_out = pos_out != 1 && inst_out == 1;
+            const int T = threads_per_instance(ap.L);
+            a.cmap = (col_in || col_out) ? 1u : 0u;
+            auto side_mode = [&](bool col, bool flat, int64_t pos) -> rk_u32 {
+                if (a.cmap) {
+                    if (col) return RK_MODE_DIRECT;
+                    if (flat) return RK_MODE_FLAT;
+                    return pos == 1 ? RK_MODE_ROW : RK_MODE_DIRECT;
+                }
+                if (flat && T < 32) return RK_MODE_FLAT;   // short rows: stream the tile linearly
+                return RK_MODE_DIRECT;
+            };
+            a.load_mode = side_mode(col_in, flat_in, pos_in);
+            a.store_mode = side_mode(col_out, flat_out, pos_out);
+            pp.C = a.cmap ? tile_c_col(ap.L) : tile_c_row(ap.L);
 
             rc = find_kernel(pp, d->flags);
             if (rc != RK_OK) return rc;
@@ -620,6 +653,8 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
             pp.grid = std::min<uint64_t>(a.n_tiles, max_grid);
             rc = build_tables(plan.get(), tb, pp, ap, n);
             if (rc != RK_OK) return rc;
+            pp.inst_per_outer = (uint64_t)(ap.H * ap.NS * I);
+            pp.outer_count = O;
             plan->passes.push_back(pp);
             cur = dst;
             cur_shape[axis] = ap.out_len;
@@ -628,6 +663,26 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
     plan->passes.front().is_first = true;
     plan->passes.back().is_last = true;
 
+    // chunked execution (see rk_fft_plan::chunk_outer)
+    if (work.size() == 1 && plan->passes.size() >= 2 && ws_elems) {
+        bool ok = true;
+        const int64_t O = plan->passes.front().outer_count;
+        for (auto& p : plan->passes)
+            ok = ok && p.loops.empty() && p.outer_count == O && O > 1 && p.inst_per_outer % (uint64_t)p.C == 0;
+        static const int64_t chunk_bytes = [] {
+            const char* e = getenv("RK_CHUNK_BYTES");
+            return e && *e ? atoll(e) : (int64_t)64 << 20;
+        }();
+        if (ok && chunk_bytes > 0) {
+            const int64_t per_outer = (int64_t)ws_elems / O;
+            const int64_t chunk = std::max<int64_t>(1, chunk_bytes / (per_outer * (int64_t)sizeof(float2)));
+            if (chunk < O) {
+                plan->chunk_outer = chunk;
+                plan->n_outer = O;
+                ws_elems = (size_t)(chunk * per_outer);
+            }
+        }
+    }
     plan->ws_bytes = 0;
     if (ws_elems) {
         const size_t one = ((ws_elems * sizeof(float2)) + 255) & ~size_t(255);
@@ -638,7 +693,7 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
         for (auto& p : plan->passes)
             if (p.src == BUF_WS1 || p.dst == BUF_WS1) plan->ws_bytes = 2 * one;
     }
-    if (!tb.bytes.empty()) {
+    if (!tb.bytes.empty() && !(d->flags & RK_FFT_DRY_RUN)) {
         RK_HIP(hipMalloc(&plan->tables, tb.bytes.size()));
         RK_HIP(hipMemcpy(plan->tables, tb.bytes.data(), tb.bytes.size(), hipMemcpyHostToDevice));
     }
@@ -646,8 +701,9 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
     return RK_OK;
 }
 
+// Launches tiles [t0, t1) of a pass (one tile per work-group).
 static int launch_pass(const PassPlan& pp, rk_pass_args& a, hipStream_t stream, const rk_user_blob* load_blob,
-                       const rk_user_blob* store_blob) {
+                       const rk_user_blob* store_blob, rk_u64 t0, rk_u64 t1) {
     // host loops over batch dims the kernel could not fold
     const size_t nl = pp.loops.size();
     int64_t total = 1;
@@ -663,10 +719,11 @@ static int launch_pass(const PassPlan& pp, rk_pass_args& a, hipStream_t stream, 
             a.in_off += idx * pp.loops[k].in_str;
             a.out_off += idx * pp.loops[k].out_str;
         }
-        // one tile per work-group; grids beyond the launch limit are split
-        for (rk_u64 base = 0; base < a.n_tiles; base += pp.grid) {
+        // grids beyond the launch limit are split; the kernel stops at tile t1
+        a.n_tiles = t1;
+        for (rk_u64 base = t0; base < t1; base += pp.grid) {
             a.tile_base = base;
-            const unsigned grid = (unsigned)std::min<rk_u64>(pp.grid, a.n_tiles - base);
+            const unsigned grid = (unsigned)std::min<rk_u64>(pp.grid, t1 - base);
             // fused kernels take (pass args, load blob, store blob)
             void* params[] = {&a, const_cast<rk_user_blob*>(load_blob), const_cast<rk_user_blob*>(store_blob)};
             if (pp.jit) {
@@ -692,6 +749,7 @@ static int plan_execute(const rk_fft_plan* plan, void* out, const void* in, int 
         c.out = (float2*)out;
         return launch_copy(c, stream);
     }
+    if (plan->desc.flags & RK_FFT_DRY_RUN) return fail(RK_ERR_INVALID, "rk_fft_execute: a dry-run plan cannot execute");
     if (plan->ws_bytes && !ws) return fail(RK_ERR_INVALID, "rk_fft_execute: plan needs %zu bytes of workspace", plan->ws_bytes);
     void* bufs[4] = {const_cast<void*>(in), out, ws, ws ? (char*)ws + plan->ws1_offset : nullptr};
     const char* tables = (const char*)plan->tables;
@@ -701,9 +759,12 @@ static int plan_execute(const rk_fft_plan* plan, void* out, const void* in, int 
     memset(&sb, 0, sizeof(sb));
     if (load_blob && load_bytes) memcpy(lb.b, load_blob, std::min<size_t>(load_bytes, sizeof(lb.b)));
     if (store_blob && store_bytes) memcpy(sb.b, store_blob, std::min<size_t>(store_bytes, sizeof(sb.b)));
-    for (size_t p = 0; p < plan->passes.size(); ++p) {
+    const size_t np = plan->passes.size();
+    std::vector<rk_pass_args> args(np);
+    for (size_t p = 0; p < np; ++p) {
         const PassPlan& pp = plan->passes[p];
-        rk_pass_args a = pp.args;
+        rk_pass_args& a = args[p];
+        a = pp.args;
         a.in = (const float2*)bufs[pp.src];
         a.out = (float2*)bufs[pp.dst];
         a.twl = (const float2*)tab(pp.twl_off);
@@ -721,10 +782,34 @@ static int plan_execute(const rk_fft_plan* plan, void* out, const void* in, int 
         } else if (norm_forward && pp.is_last) {
             a.out_scale = (float)plan->norm;
         }
-        if (plan->timing) RK_HIP(hipEventRecord(plan->events[2 * p], stream));
-        int rc = launch_pass(pp, a, stream, &lb, &sb);
-        if (rc != RK_OK) return rc;
-        if (plan->timing) RK_HIP(hipEventRecord(plan->events[2 * p + 1], stream));
+    }
+    if (!plan->chunk_outer) {
+        for (size_t p = 0; p < np; ++p) {
+            if (plan->timing) RK_HIP(hipEventRecord(plan->events[2 * p], stream));
+            int rc = launch_pass(plan->passes[p], args[p], stream, &lb, &sb, 0, args[p].n_tiles);
+            if (rc != RK_OK) return rc;
+            if (plan->timing) RK_HIP(hipEventRecord(plan->events[2 * p + 1], stream));
+        }
+        return RK_OK;
+    }
+    // chunked: all passes over `chunk_outer` outer indices, then the next chunk;
+    // the workspace side of each pass is rebased to the chunk
+    int64_t chunk_idx = 0;
+    for (int64_t o0 = 0; o0 < plan->n_outer; o0 += plan->chunk_outer, ++chunk_idx) {
+        const int64_t o1 = std::min(plan->n_outer, o0 + plan->chunk_outer);
+        for (size_t p = 0; p < np; ++p) {
+            const PassPlan& pp = plan->passes[p];
+            rk_pass_args a = args[p];
+            if (pp.src == BUF_WS0 || pp.src == BUF_WS1) a.in_off -= o0 * a.in_so;
+            if (pp.dst == BUF_WS0 || pp.dst == BUF_WS1) a.out_off -= o0 * a.out_so;
+            const rk_u64 t0 = (rk_u64)o0 * pp.inst_per_outer / (rk_u64)pp.C;
+            const rk_u64 t1 = (rk_u64)o1 * pp.inst_per_outer / (rk_u64)pp.C;
+            const size_t ev = 2 * (chunk_idx * np + p);
+            if (plan->timing && ev + 1 < plan->events.size()) RK_HIP(hipEventRecord(plan->events[ev], stream));
+            int rc = launch_pass(pp, a, stream, &lb, &sb, t0, t1);
+            if (rc != RK_OK) return rc;
+            if (plan->timing && ev + 1 < plan->events.size()) RK_HIP(hipEventRecord(plan->events[ev + 1], stream));
+        }
     }
     return RK_OK;
 }
@@ -814,7 +899,9 @@ extern "C" int rk_fft_plan_set_pass_kernel(rk_fft_plan* plan, int pass, void* fu
 extern "C" int rk_fft_plan_set_timing(rk_fft_plan* plan, int enable) {
     if (!plan) return fail(RK_ERR_INVALID, "rk_fft_plan_set_timing: null plan");
     if (enable && plan->events.empty()) {
-        plan->events.resize(2 * plan->passes.size());
+        const int64_t chunks =
+            plan->chunk_outer ? (plan->n_outer + plan->chunk_outer - 1) / plan->chunk_outer : 1;
+        plan->events.resize(2 * plan->passes.size() * (size_t)chunks);
         for (auto& e : plan->events) RK_HIP(hipEventCreate(&e));
     }
     plan->timing = enable != 0;
@@ -824,8 +911,16 @@ extern "C" int rk_fft_plan_set_timing(rk_fft_plan* plan, int enable) {
 extern "C" int rk_fft_plan_pass_time_ms(const rk_fft_plan* plan, int pass, float* ms) {
     if (!plan || !ms) return fail(RK_ERR_INVALID, "rk_fft_plan_pass_time_ms: null argument");
     *ms = -1.0f;
-    if (!plan->timing || pass < 0 || pass >= (int)plan->passes.size()) return RK_OK;
-    RK_HIP(hipEventSynchronize(plan->events[2 * pass + 1]));
-    RK_HIP(hipEventElapsedTime(ms, plan->events[2 * pass], plan->events[2 * pass + 1]));
+    const size_t np = plan->passes.size();
+    if (!plan->timing || pass < 0 || pass >= (int)np) return RK_OK;
+    // sum over the chunks the pass ran in
+    float total = 0.0f;
+    for (size_t ev = 2 * (size_t)pass; ev + 1 < plan->events.size(); ev += 2 * np) {
+        float t = 0.0f;
+        RK_HIP(hipEventSynchronize(plan->events[ev + 1]));
+        RK_HIP(hipEventElapsedTime(&t, plan->events[ev], plan->events[ev + 1]));
+        total += t;
+    }
+    *ms = total;
     return RK_OK;
 }

@@ -328,6 +328,14 @@ extern "C" int rk_module_compile(const char* source, const char* options, void**
     return RK_OK;
 }
 
+extern "C" int rk_module_check(const char* source, const char* options, char* log, size_t log_size) {
+    if (!source) return fail(RK_ERR_INVALID, "rk_module_check: null source");
+    std::string code, lg;
+    int rc = rk::compile_code_object(source, {}, {}, options ? options : "", &code, &lg);
+    if (log && log_size) snprintf(log, log_size, "%s", lg.c_str());
+    return rc;
+}
+
 extern "C" int rk_module_get_function(void* module, const char* name, void** function) {
     hipFunction_t f;
     RK_HIP(hipModuleGetFunction(&f, (hipModule_t)module, name));

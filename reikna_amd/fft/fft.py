@@ -103,6 +103,8 @@ class FFT(Computation):
         if fused_out:
             out_meta = ArrayMetadata(out_meta.shape, out_meta.dtype)
         flags = self._flags | (_lib.RK_FFT_OUT_LAST_ONLY if fused_out else 0)
+        if plan.virtual:
+            flags |= _lib.RK_FFT_DRY_RUN
         native = NativeFftPlan(in_meta, out_meta, self._axes, flags)
         self.native_plan = native
         ws = plan.temp_array((native.workspace_bytes,), numpy.uint8) if native.workspace_bytes else None
@@ -166,8 +168,9 @@ class _FusedFft:
             src = self._source(info, slices, argobjs.get(input_.name) if "load" in sides else None,
                                argobjs.get(output.name) if "store" in sides else None)
             self.sources[index] = src
-            fn = compile_kernel(src, "rk_fused_pass", options)
-            native.set_pass_kernel(index, fn)
+            fn = compile_kernel(src, "rk_fused_pass", options, check_only=plan.virtual)
+            if not plan.virtual:
+                native.set_pass_kernel(index, fn)
 
     def _source(self, info: Any, slices: tuple[int, int, int], in_obj: Any, out_obj: Any) -> str:
         def struct(name: str, leaves: list[Parameter]) -> str:
@@ -189,6 +192,7 @@ class _FusedFft:
                 store_body = bind("sa", self.store_leaves) + f"        {mod}((long long)o, (long long)k, (long long)i, v);\n"
             parts.append(
                 "struct RkFusedIO {\n"
+                "    static constexpr bool kBranchFree = false;\n"
                 "    const RkLoadArgs& la;\n"
                 "    const RkStoreArgs& sa;\n"
                 "    __device__ __forceinline__ float2 load(const rk_pass_args& a, rk_u64 o, rk_u64 m, rk_u64 i,"

@@ -248,6 +248,33 @@ class Device:
         return f"Device({self.index}: {self.name}, {self.arch})"
 
 
+class VirtualDevice:
+    """
+    A stand-in device for planning without a GPU: ``compile()`` against it runs
+    the planners (native plans in dry-run mode) and compiles every generated
+    kernel for gfx950 with hiprtc, but allocates nothing and cannot execute.
+    Used by the CPU test-suite to cover code generation.
+    """
+
+    virtual = True
+
+    def __init__(self, params: DeviceParameters | None = None):
+        self.index = -1
+        self.params = params or DeviceParameters()
+        self.name = self.params.name
+        self.arch = self.params.arch
+
+    @property
+    def device(self) -> VirtualDevice:
+        return self
+
+    def activate(self) -> None:
+        pass
+
+    def __repr__(self) -> str:
+        return f"VirtualDevice({self.arch})"
+
+
 def default_device() -> Device:
     return Device(0)
 

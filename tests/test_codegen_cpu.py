@@ -1,0 +1,122 @@
+"""Planner structure and generated-kernel compilation without a GPU.
+
+Compiling against a VirtualDevice runs the native planner in dry-run mode
+(RK_FFT_DRY_RUN) and compiles every generated kernel for gfx950 with hiprtc,
+so the pass chains the reference's planner would build (reikna/fft/fft.py:487-584)
+and every fused-transformation kernel are checked on the CPU.
+"""
+
+import numpy
+import pytest
+
+from reikna_amd import ArrayMetadata
+from reikna_amd.algorithms import PureParallel
+from reikna_amd.fft import FFT
+from reikna_amd.runtime import VirtualDevice
+from reikna_amd.transformations import combine_complex, copy, div_const, mul_param, norm_const, split_complex
+
+
+@pytest.fixture(scope="module")
+def vdev():
+    return VirtualDevice()
+
+
+def passes(shape, axes, flags=0, dev=None):
+    fft = FFT(ArrayMetadata(shape, numpy.complex64), axes=axes, flags=flags)
+    fft.compile(dev or VirtualDevice())
+    return fft.native_plan.passes
+
+
+def test_plan_single_pass_rows(vdev):
+    ps = passes((1024, 1024), (1,), dev=vdev)
+    assert [(p.L, p.load_mode, p.store_mode) for p in ps] == [(1024, "direct", "direct")]
+
+
+def test_plan_two_pass_large_axis(vdev):
+    ps = passes((64, 2**20), (1,), dev=vdev)
+    assert [p.L for p in ps] == [1024, 1024]
+    assert ps[0].load_mode == "direct" and ps[0].store_mode == "flat"     # column load, contiguous store
+    assert ps[1].load_mode == "direct" and ps[1].store_mode == "direct"   # column both sides
+    assert ps[1].variant & 3 == 1                                        # inter-pass twiddle variant
+    assert ps[0].src == "in" and ps[0].dst == "ws0" and ps[1].src == "ws0" and ps[1].dst == "out"
+
+
+def test_plan_three_axes(vdev):
+    ps = passes((256, 256, 256), (0, 1, 2), dev=vdev)
+    assert [(p.axis, p.L) for p in ps] == [(2, 256), (1, 256), (0, 256)]
+    assert all(p.dst == "out" for p in ps)   # single-pass axes run in place on `out`
+
+
+@pytest.mark.parametrize("n,expect", [(997, 2048), (1009, 2048), (5003, 10240), (12289, 32768), (65521, 131072)])
+def test_plan_bluestein_length(vdev, n, expect):
+    ps = passes((4, n), (1,), dev=vdev)
+    assert all(p.axis_len == expect for p in ps)
+    if expect <= 16384:
+        assert len(ps) == 1 and ps[0].kind == 1   # single-kernel Bluestein
+    else:
+        assert len(ps) == 4                       # forward chain + second chain
+
+
+def test_plan_smooth_non_pow2_is_not_bluestein(vdev):
+    for n in (1000, 3 * 2**12, 5 * 2**11, 7 * 2**9):
+        ps = passes((4, n), (1,), dev=vdev)
+        assert len(ps) == 1 and ps[0].kind == 0 and ps[0].L == n
+
+
+def test_plan_flags(vdev):
+    import reikna_amd._lib as L
+
+    assert len(passes((8, 1024), (1,), L.RK_FFT_FORCE_MULTIPASS, dev=vdev)) == 2
+    ps = passes((8, 1024), (1,), L.RK_FFT_FORCE_BLUESTEIN, dev=vdev)
+    assert ps[0].kind == 1 and ps[0].axis_len == 2048
+
+
+def test_dry_plan_cannot_execute(vdev):
+    fft = FFT(ArrayMetadata((4, 64), numpy.complex64), axes=(1,))
+    fft.compile(vdev)
+    with pytest.raises(ValueError, match="dry-run"):
+        fft.native_plan.execute(1, 1, 0, None, None)
+
+
+# -- generated kernels ---------------------------------------------------------
+def test_fused_input_and_output_compile(vdev):
+    arr = ArrayMetadata((64, 4096), numpy.complex64)
+    fft = FFT(arr, axes=(1,))
+    scale = mul_param(arr, numpy.float32)
+    fft.parameter.input.connect(scale, scale.output, input_prime=scale.input, param=scale.param)
+    div = div_const(arr, 8.0)
+    fft.parameter.output.connect(div, div.input, output_prime=div.output)
+    assert list(fft.signature.parameters) == ["output_prime", "input_prime", "param", "inverse"]
+    fft.compile(vdev)
+
+
+def test_fused_multi_pass_and_multi_axis_compile(vdev):
+    for shape, axes in (((4, 2**16), (1,)), ((32, 64, 48), (0, 1, 2)), ((8, 997), (1,)), ((2, 65521), (1,))):
+        arr = ArrayMetadata(shape, numpy.complex64)
+        fft = FFT(arr, axes=axes)
+        real = ArrayMetadata(shape, numpy.float32)
+        comb = combine_complex(arr)
+        fft.parameter.input.connect(comb, comb.output, re=comb.real, im=comb.imag)
+        nrm = norm_const(arr, 2)
+        fft.parameter.output.connect(nrm, nrm.input, power=nrm.output)
+        assert list(fft.signature.parameters) == ["power", "re", "im", "inverse"]
+        assert fft.signature.parameters["re"].annotation.array_metadata == real
+        fft.compile(vdev)
+
+
+def test_split_complex_output(vdev):
+    arr = ArrayMetadata((16, 256), numpy.complex64)
+    fft = FFT(arr, axes=(1,))
+    sp = split_complex(arr)
+    fft.parameter.output.connect(sp, sp.input, re=sp.real, im=sp.imag)
+    assert list(fft.signature.parameters) == ["re", "im", "input", "inverse"]
+    fft.compile(vdev)
+
+
+def test_pure_parallel_from_transformation_compiles(vdev):
+    arr = ArrayMetadata((100, 30), numpy.complex64)
+    trf = copy(arr)
+    pp = PureParallel.from_trf(trf, trf.input)
+    c = pp.compile(vdev)
+    assert len(c.kernel_sources) == 1
+    assert "rk_skip" in c.kernel_sources[0]
