@@ -51,6 +51,8 @@ struct rk_pass_args {
     rk_u64 n_inst;
     rk_u64 n_tiles;
     rk_u64 tile_base;        // first tile of this launch (grids are split on the host)
+    rk_u32 xcd_group;        // >1: consecutive tiles in groups of this size run on one XCD
+    rk_u32 n_xcd;            // XCDs the dispatcher round-robins work-groups over
     rk_u64 dim_I, dim_NS, dim_H;
     rk_i64 in_off, in_so, in_si, in_sa;
     rk_i64 out_off, out_so, out_si, out_sa;

@@ -523,6 +523,21 @@ __device__ __forceinline__ void run_stages(float2 (&v)[VM], void* my, int t, con
 // ------------------------------------------------------------------------
 // the pass kernel body
 // ------------------------------------------------------------------------
+// Tile of this work-group.  The dispatcher hands work-group b to XCD
+// b % n_xcd; with xcd_group = G > 1 the G tiles of each group of consecutive
+// tiles go to work-groups on one XCD, so tiles that share cache lines (narrow
+// column tiles) share that XCD's L2 instead of each fetching whole lines.
+__device__ __forceinline__ rk_u64 xcd_tile(const rk_pass_args& a) {
+    const rk_u32 b = blockIdx.x;
+    const rk_u32 g = a.xcd_group, nx = a.n_xcd;
+    if (g <= 1) return b;
+    const rk_u32 span = g * nx;
+    const rk_u32 full = (gridDim.x / span) * span;
+    if (b >= full) return b;
+    const rk_u32 x = b % nx, k = b / nx;
+    return (rk_u64)(k / g) * span + x * g + (k % g);
+}
+
 // LDS bytes of a tile and whether its exchanges are split into halves.
 template <int L, int C> __host__ __device__ constexpr bool tile_split() {
     return C * lds_pitch<L>() * 8 > 80 * 1024;
@@ -547,7 +562,7 @@ __device__ __forceinline__ void fft_pass_body(const rk_pass_args& a, const IO& i
     constexpr bool SPLIT = tile_split<L, C>();
     __shared__ float2 lds[SPLIT ? (C * LP + 1) / 2 : C * LP];
 
-    const rk_u64 tile = (rk_u64)blockIdx.x + a.tile_base;
+    const rk_u64 tile = xcd_tile(a) + a.tile_base;
     if (tile >= a.n_tiles) return;
     const rk_u64 inst0 = tile * (rk_u64)C;
 

@@ -30,8 +30,55 @@ using rk::fail;
 
 namespace rk {
 
-static constexpr int64_t kSingleMax = 16384;   // longest single-pass DFT
-static constexpr int64_t kFactorMax = 4096;    // longest DFT of one pass in a multi-pass chain
+// Longest single-pass DFT (RK_SINGLE_MAX overrides, at most 16384).
+static int64_t single_max() {
+    static const int64_t v = [] {
+        const char* e = getenv("RK_SINGLE_MAX");
+        const int64_t x = e && *e ? (int64_t)atoll(e) : (int64_t)16384;
+        return std::min<int64_t>(std::max<int64_t>(x, 2), 16384);
+    }();
+    return v;
+}
+#define kSingleMax (single_max())
+
+// Longest DFT of one pass in a multi-pass chain.  Column-style passes need
+// 16 adjacent instances per tile for 128-byte accesses, so a pass of length L
+// holds a 16 x L tile in LDS: up to L = 256 (32 KB) several work-groups share
+// a CU and every pass streams at copy speed, while L = 1024 tiles (128 KB)
+// leave one work-group per CU and run ~60% slower.  RK_FACTOR_MAX overrides.
+static int64_t factor_max() {
+    static const int64_t v = [] {
+        const char* e = getenv("RK_FACTOR_MAX");
+        return e && *e ? (int64_t)atoll(e) : (int64_t)256;
+    }();
+    return v;
+}
+
+// Split n into exactly p smooth factors <= fmax, the first as close to
+// n^(1/p) as possible (balanced factors keep every tile small).
+static bool split_into(int64_t n, int p, int64_t fmax, std::vector<int64_t>& out) {
+    if (p == 1) {
+        if (n <= fmax && is_smooth(n)) {
+            out.push_back(n);
+            return true;
+        }
+        return false;
+    }
+    const double target = std::pow((double)n, 1.0 / p);
+    std::vector<int64_t> cands;
+    for (int64_t d = 2; d <= fmax && d <= n; ++d)
+        if (n % d == 0 && is_smooth(d)) cands.push_back(d);
+    std::sort(cands.begin(), cands.end(), [&](int64_t a, int64_t b) {
+        return std::fabs(std::log((double)a / target)) < std::fabs(std::log((double)b / target));
+    });
+    for (int64_t d : cands) {
+        const size_t keep = out.size();
+        out.push_back(d);
+        if (split_into(n / d, p - 1, fmax, out)) return true;
+        out.resize(keep);
+    }
+    return false;
+}
 
 enum BufId { BUF_IN = 0, BUF_OUT = 1, BUF_WS0 = 2, BUF_WS1 = 3 };
 
@@ -209,31 +256,22 @@ namespace rk {
 // planning helpers
 // ---------------------------------------------------------------------------
 
-// Split a smooth n into pass lengths, each <= kFactorMax, as balanced as possible.
+// Split a smooth n into pass lengths, each <= factor_max(), as balanced as possible.
 static bool split_factors(int64_t n, std::vector<int64_t>& out) {
     out.clear();
-    if (n <= kFactorMax) {
+    const int64_t fmax = factor_max();
+    if (n <= fmax) {
         out.push_back(n);
         return true;
     }
-    // two factors: the divisor d <= sqrt(n) that is smooth and maximises min(d, n/d)
-    int64_t best = 0;
-    for (int64_t d = 2; d * d <= n; ++d)
-        if (n % d == 0 && is_smooth(d) && n / d <= kFactorMax) best = d;
-    if (best) {
-        out.push_back(best);
-        out.push_back(n / best);
-        return true;
+    // the fewest passes whose lengths can all stay <= fmax
+    for (int p = 2; p <= 8; ++p) {
+        double cap = 1.0;
+        for (int k = 0; k < p; ++k) cap *= (double)fmax;
+        if (cap < (double)n) continue;
+        if (split_into(n, p, fmax, out)) return true;
+        out.clear();
     }
-    // otherwise peel the largest factor <= kFactorMax that keeps a valid rest
-    for (int64_t d = kFactorMax; d >= 2; --d)
-        if (n % d == 0 && is_smooth(d)) {
-            std::vector<int64_t> rest;
-            if (!split_factors(n / d, rest)) continue;
-            out.push_back(d);
-            out.insert(out.end(), rest.begin(), rest.end());
-            return true;
-        }
     return false;
 }
 
@@ -671,7 +709,10 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
             ok = ok && p.loops.empty() && p.outer_count == O && O > 1 && p.inst_per_outer % (uint64_t)p.C == 0;
         static const int64_t chunk_bytes = [] {
             const char* e = getenv("RK_CHUNK_BYTES");
-            return e && *e ? atoll(e) : (int64_t)64 << 20;
+            // off by default: on MI355X chunks of 32-128 MiB measured no faster
+            // than whole-batch passes (the launches per chunk cost more than
+            // the Infinity Cache saves)
+            return e && *e ? atoll(e) : (int64_t)0;
         }();
         if (ok && chunk_bytes > 0) {
             const int64_t per_outer = (int64_t)ws_elems / O;

@@ -32,13 +32,17 @@ def test_plan_single_pass_rows(vdev):
     assert [(p.L, p.load_mode, p.store_mode) for p in ps] == [(1024, "direct", "direct")]
 
 
-def test_plan_two_pass_large_axis(vdev):
+def test_plan_multi_pass_large_axis(vdev):
     ps = passes((64, 2**20), (1,), dev=vdev)
-    assert [p.L for p in ps] == [1024, 1024]
+    # fewest passes with every pass length <= 256 (16-column tiles of <= 32 KB)
+    assert len(ps) == 3 and numpy.prod([p.L for p in ps]) == 2**20 and max(p.L for p in ps) <= 256
     assert ps[0].load_mode == "direct" and ps[0].store_mode == "flat"     # column load, contiguous store
-    assert ps[1].load_mode == "direct" and ps[1].store_mode == "direct"   # column both sides
-    assert ps[1].variant & 3 == 1                                        # inter-pass twiddle variant
-    assert ps[0].src == "in" and ps[0].dst == "ws0" and ps[1].src == "ws0" and ps[1].dst == "out"
+    for p in ps[1:]:
+        assert p.load_mode == "direct" and p.store_mode == "direct"     # column both sides
+        assert p.variant & 3 == 1                                        # inter-pass twiddle variant
+    assert [(p.src, p.dst) for p in ps] == [("in", "ws0"), ("ws0", "ws1"), ("ws1", "out")]
+    ps = passes((64, 2**16), (1,), dev=vdev)
+    assert [p.L for p in ps] == [256, 256]
 
 
 def test_plan_three_axes(vdev):
