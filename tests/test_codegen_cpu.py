@@ -58,7 +58,11 @@ def test_plan_bluestein_length(vdev, n, expect):
     if expect <= 16384:
         assert len(ps) == 1 and ps[0].kind == 1   # single-kernel Bluestein
     else:
-        assert len(ps) == 4                       # forward chain + second chain
+        # forward chain over M, then a second forward chain of the same passes
+        assert len(ps) % 2 == 0 and len(ps) >= 4
+        half = len(ps) // 2
+        assert [p.L for p in ps[:half]] == [p.L for p in ps[half:]]
+        assert ps[0].pre_ops == 1 and ps[half - 1].post_ops == 1 and ps[-1].post_ops == 6
 
 
 def test_plan_smooth_non_pow2_is_not_bluestein(vdev):
