@@ -37,6 +37,18 @@ WORKLOADS = {
     "c3-1000": ((4096, 1000), (1,), "uniform", 4, "c64 C2C FFT 4096 x 1000 over axis 1"),
     "c3-997": ((4096, 997), (1,), "uniform", 4, "c64 C2C FFT 4096 x 997 (prime, Bluestein) over axis 1"),
     "c3-12288": ((4096, 12288), (1,), "uniform", 4, "c64 C2C FFT 4096 x 12288 over axis 1"),
+    # batched uneven lengths: handled by bench_batch
+    "c4": (None, None, "uniform", 6, "512 c64 C2C transforms, lengths log-uniform in [64, 65536] "
+           "(2^k, 3*2^k, 5*2^k, primes; seed 5), fused 1/N, one launch"),
+}
+
+# HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE runs of this benchmark (profiles/fft_pass_c2b_64x2p20.md):
+# FETCH_SIZE reads half of a copy kernel's known read size on this GPU, so
+# traffic = 2 * FETCH_SIZE + WRITE_SIZE (KiB per dispatch).
+MEASURED_TRAFFIC = {
+    # (workload, L, C, variant of the dominant pass): bytes per launch
+    ("c2b", 128, 32, 1): (2 * 262298.0 + 524288.0) * 1024,
 }
 
 
@@ -91,6 +103,7 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2b", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--autotune", action="store_true", help="c4: pick batch planner knobs by measurement")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(
@@ -107,6 +120,8 @@ def main(argv=None) -> int:
     from reikna_amd.fft import FFT
     from reikna_amd.runtime import Device
 
+    if args.workload == "c4":
+        return bench_batch(args)
     shape, axes, dist, seed, desc = WORKLOADS[args.workload]
     dev = Device(0)
     queue = Queue(dev)
@@ -200,7 +215,7 @@ def main(argv=None) -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC.get((args.workload, passes[dom].L, passes[dom].C, passes[dom].variant)),
             "kernel": f"pass {dom} (L={passes[dom].L}), {pass_ms[dom] * 1e3:.1f} us avg",
             "transform_GBs": round(transform_gbs, 1),
             "transform_frac": round(transform_gbs / HBM_PEAK_GBS, 4),
@@ -217,6 +232,136 @@ def main(argv=None) -> int:
         out["cpu_baseline"] = cpu_baseline(shape, axes, data)
     print(json.dumps(out), flush=True)
     return 0 if err <= tol else 1
+
+
+def bench_batch(args) -> int:
+    """512 transforms of mixed lengths in one launch of the batch kernel,
+    against the per-item path (one plan and its launches per transform)."""
+    import numpy
+
+    import oracle
+    from reikna_amd import Array, Event, Queue
+    from reikna_amd.fft import BatchFFT, BatchOptions
+    from reikna_amd.fft.batch import packed_items, sample_lengths
+    from reikna_amd.runtime import Device
+
+    _, _, dist, seed, desc = WORKLOADS["c4"]
+    dev = Device(0)
+    queue = Queue(dev)
+    lengths = sample_lengths(512, 64, 65536, seed=5)
+    items = packed_items(lengths, scale=True)
+    total = sum(lengths)
+    data = oracle.test_array((total,), seed=seed, dist=dist)
+    inp = Array.from_host(queue, data)
+    out = Array.from_host(queue, numpy.zeros_like(data))
+    print(f"[bench] {desc} on {dev.name}: {total} elements", file=sys.stderr, flush=True)
+    opts = BatchOptions()
+    if args.autotune:
+        from reikna_amd.fft.autotune import autotune_batch
+
+        opts, _ = autotune_batch(dev, items, queue)
+        print(f"[bench] autotuned options: {opts}", file=sys.stderr, flush=True)
+    batch = BatchFFT(dev, items, opts)
+    info = batch.info
+
+    def timed(fn, steps):
+        for _ in range(args.warmup):
+            fn()
+        queue.synchronize()
+        e0, e1 = Event(), Event()
+        e0.record(queue)
+        for _ in range(steps):
+            fn()
+        e1.record(queue)
+        queue.synchronize()
+        return e0.elapsed_ms(e1) / steps
+
+    ms = timed(lambda: batch(queue, out, inp), args.steps)
+    # per-launch durations (the queue kernel is the only launch)
+    single = []
+    for _ in range(max(5, args.steps // 2)):
+        e0, e1 = Event(), Event()
+        e0.record(queue)
+        batch(queue, out, inp)
+        e1.record(queue)
+        e1.synchronize()
+        single.append(e0.elapsed_ms(e1))
+    kernel_ms = sorted(single)[len(single) // 2]
+    batch.check()
+    res = out.get(queue)
+    worst = 0.0
+    for it in items:
+        x = data[it.in_offset: it.in_offset + it.length]
+        ref = oracle.fftn(x, (0,)) / it.length
+        worst = max(worst, oracle.rel_l2(res[it.out_offset: it.out_offset + it.length], ref)
+                    / oracle.tolerance((it.length,)))
+    # HIP graph replay of the same launch
+    with queue.capture() as cap:
+        batch(queue, out, inp)
+    graph_ms = timed(lambda: cap.graph.launch(queue), args.steps)
+    # the per-item path
+    grouped = BatchFFT(dev, items, BatchOptions(mode="grouped"))
+    grouped_ms = timed(lambda: grouped(queue, out, inp), max(3, args.steps // 4))
+
+    flops = sum(5.0 * n * math.log2(n) for n in lengths)
+    alg_bytes = 16.0 * total
+    achieved = alg_bytes / (kernel_ms / 1e3) / 1e9
+    out_json = {
+        "metric": METRIC,
+        "value": round(flops / (ms / 1e3) / 1e9, 1),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {
+            "workload": desc,
+            "transforms": len(lengths),
+            "elements": total,
+            "distribution": f"{dist}, seed {seed}",
+            "options": {k: getattr(opts, k) for k in ("single_max", "factor_max", "blue_max", "order")},
+            "plan": info,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": f"rk_fft_batch, {kernel_ms * 1e3:.1f} us median",
+        },
+        "graph_ms_per_step": round(graph_ms, 5),
+        "grouped_ms_per_step": round(grouped_ms, 5),
+        "speedup_vs_grouped": round(grouped_ms / ms, 2),
+        "worst_rel_l2_over_bound": round(worst, 4),
+    }
+    if not args.no_cpu:
+        print("[bench] CPU baseline ...", file=sys.stderr, flush=True)
+        import scipy.fft as sfft
+
+        workers = len(os.sched_getaffinity(0))
+        times = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            for it in items:
+                sfft.fft(data[it.in_offset: it.in_offset + it.length], norm="forward", workers=workers)
+            times.append(time.perf_counter() - t0)
+        out_json["cpu_baseline"] = {
+            "value": round(flops / min(times) / 1e9, 2),
+            "unit": "GFLOP/s",
+            "cores": workers,
+            "kind": "port",
+            "sample": f"scipy.fft {__import__('scipy').__version__} (pocketfft), all 512 transforms one by one, "
+            f"min of {len(times)} runs",
+        }
+    print(json.dumps(out_json), flush=True)
+    return 0 if worst <= 1.0 else 1
 
 
 if __name__ == "__main__":

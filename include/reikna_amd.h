@@ -222,8 +222,9 @@ int rk_fft_plan_set_timing(rk_fft_plan* plan, int enable);
 int rk_fft_plan_pass_time_ms(const rk_fft_plan* plan, int pass, float* ms);
 
 /* ------------------------------------------------------------------ */
-/* Batched uneven transforms — one persistent launch (no reference      */
-/* counterpart: reikna compiles one computation per shape).             */
+/* Batched uneven transforms — the whole batch in one kernel launch     */
+/* (no reference counterpart: reikna compiles and launches one          */
+/* computation per shape, reikna/fft/fft.py:723-756).                   */
 /* ------------------------------------------------------------------ */
 
 typedef struct rk_batch_plan rk_batch_plan;
@@ -240,15 +241,60 @@ typedef struct {
 
 #define RK_BATCH_SCALE 0x1u
 
+/* Execution modes: one launch of the work-queue kernel for the whole batch
+   (default), or one single-shape plan per item (the per-request path). */
+#define RK_BATCH_MODE_QUEUE 0u
+#define RK_BATCH_MODE_GROUPED 1u
+
+#define RK_BATCH_DRY_RUN 0x1u   /* plan only: no device memory, cannot execute */
+
+typedef struct {
+    uint32_t mode;        /* RK_BATCH_MODE_* */
+    uint32_t flags;       /* RK_BATCH_DRY_RUN */
+    int32_t single_max;   /* longest single-pass length (0: 4096, at most 4096) */
+    int32_t factor_max;   /* longest pass of a multi-pass chain (0: 256, at most 256) */
+    int32_t blue_max;     /* longest single-kernel Bluestein length (0: 2048, at most 2048) */
+    int32_t order;        /* queue order inside a level: 0 largest tiles first, 1 item order */
+} rk_batch_options;
+
+typedef struct {
+    int64_t items;        /* transforms in the batch (sum of counts) */
+    int64_t passes;       /* passes over all items */
+    int64_t tiles;        /* work-groups of the queue launch */
+    int64_t workspace_bytes;
+    int32_t levels;       /* longest pass chain */
+    int32_t launches;     /* kernel launches per execute */
+    int32_t mode;
+    int32_t lds_bytes;    /* LDS per work-group of the queue kernel */
+} rk_batch_info;
+
 int rk_batch_plan_create(const rk_batch_item* items, int n_items,
                          rk_batch_plan** plan);
+int rk_batch_plan_create_ex(const rk_batch_item* items, int n_items,
+                            const rk_batch_options* options, rk_batch_plan** plan);
 int rk_batch_plan_workspace_size(const rk_batch_plan* plan, size_t* bytes);
+/* Asynchronous on `stream`; a plan's queue state is shared by its
+   executions, so executions of one plan must not overlap. */
 int rk_batch_execute(const rk_batch_plan* plan, void* out, const void* in,
                      int inverse, void* workspace, void* stream);
 int rk_batch_plan_destroy(rk_batch_plan* plan);
 /* Number of work items / launches the batch plan uses (introspection). */
 int rk_batch_plan_stats(const rk_batch_plan* plan, int64_t* work_items,
                         int* launches);
+int rk_batch_plan_info(const rk_batch_plan* plan, rk_batch_info* info);
+/* Synchronous: 1 in *hit if a dependency wait of an earlier execute gave up
+   after its poll limit (the results of that execute are not valid); clears
+   the flag. */
+int rk_batch_plan_check(rk_batch_plan* plan, int* hit);
+
+/* ------------------------------------------------------------------ */
+/* HIP graphs: capture a stream's work once, replay it with one call.   */
+/* ------------------------------------------------------------------ */
+typedef struct rk_graph rk_graph;
+int rk_graph_begin(void* stream);
+int rk_graph_end(void* stream, rk_graph** graph);
+int rk_graph_launch(rk_graph* graph, void* stream);
+int rk_graph_destroy(rk_graph* graph);
 
 #ifdef __cplusplus
 }

@@ -115,6 +115,35 @@ class BatchItem(ctypes.Structure):
     ]
 
 
+class BatchOptions(ctypes.Structure):
+    _fields_ = [
+        ("mode", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
+        ("single_max", ctypes.c_int32),
+        ("factor_max", ctypes.c_int32),
+        ("blue_max", ctypes.c_int32),
+        ("order", ctypes.c_int32),
+    ]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("items", ctypes.c_int64),
+        ("passes", ctypes.c_int64),
+        ("tiles", ctypes.c_int64),
+        ("workspace_bytes", ctypes.c_int64),
+        ("levels", ctypes.c_int32),
+        ("launches", ctypes.c_int32),
+        ("mode", ctypes.c_int32),
+        ("lds_bytes", ctypes.c_int32),
+    ]
+
+
+RK_BATCH_SCALE = 0x1
+RK_BATCH_MODE_QUEUE = 0
+RK_BATCH_MODE_GROUPED = 1
+RK_BATCH_DRY_RUN = 0x1
+
 _P = ctypes.c_void_p
 _PP = ctypes.POINTER(ctypes.c_void_p)
 _I = ctypes.c_int
@@ -168,6 +197,13 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "rk_batch_execute": (_I, [_P, _P, _P, _I, _P, _P]),
     "rk_batch_plan_destroy": (_I, [_P]),
     "rk_batch_plan_stats": (_I, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(_I)]),
+    "rk_batch_plan_create_ex": (_I, [ctypes.POINTER(BatchItem), _I, ctypes.POINTER(BatchOptions), _PP]),
+    "rk_batch_plan_info": (_I, [_P, ctypes.POINTER(BatchInfo)]),
+    "rk_batch_plan_check": (_I, [_P, ctypes.POINTER(_I)]),
+    "rk_graph_begin": (_I, [_P]),
+    "rk_graph_end": (_I, [_P, _PP]),
+    "rk_graph_launch": (_I, [_P, _P]),
+    "rk_graph_destroy": (_I, [_P]),
 }
 
 _lock = threading.Lock()

@@ -132,6 +132,57 @@ def catalog_entries() -> list[tuple[int, int, int]]:
     return sorted(ents)
 
 
+# persistent batch kernel: fixed work-group size, one case per (L, C, V)
+BATCH_THREADS = 256
+BATCH_SINGLE_MAX = 4096
+BATCH_FACTOR_MAX = 256
+
+
+def batch_lengths() -> list[int]:
+    ls = {b * 2**k for b in (1, 3, 5) for k in range(0, 13)}
+    return sorted(x for x in ls if 2 <= x <= BATCH_SINGLE_MAX)
+
+
+def batch_tile_c(length: int) -> int:
+    return max(1, BATCH_THREADS // threads_per_instance(length))
+
+
+BATCH_BLUE_MAX = 2048
+
+
+def batch_entries() -> list[tuple[int, int, int]]:
+    """Tile cases of the batch kernel.  Bluestein runs over power-of-two
+    lengths only (single kernel up to BATCH_BLUE_MAX, two chains above), and
+    multi-pass chains use pass lengths 16..BATCH_FACTOR_MAX."""
+    ents = set()
+    for length in batch_lengths():
+        c = batch_tile_c(length)
+        pow2 = length & (length - 1) == 0
+        ents.add((length, c, VAR_PLAIN))
+        if pow2 and 8 <= length <= BATCH_BLUE_MAX:
+            ents.add((length, c, VAR_BLUE))
+        if 16 <= length <= BATCH_FACTOR_MAX:
+            ents.add((length, c, VAR_TWIDDLE))
+            if pow2:
+                for v in (VAR_CHIRP, VAR_TWIDDLE_KMUL, VAR_TWIDDLE_CONJ_CHIRP):
+                    ents.add((length, c, v))
+    return sorted(ents)
+
+
+def generate_batch_cases(out_dir: Path) -> Path:
+    lines = [
+        "// generated by reikna_amd/csrc/build.py: tile cases of the persistent batch kernel",
+        "// X(case id, L, C, variant)",
+        "#define RK_BATCH_CASES(X) \\",
+    ]
+    for k, (length, c, v) in enumerate(batch_entries()):
+        lines.append(f"    X({k}, {length}, {c}, {v}) \\")
+    lines.append("")
+    path = out_dir / "batch_cases.inc"
+    write_if_changed(path, "\n".join(lines) + "\n")
+    return path
+
+
 def variant_cost(v: int) -> float:
     return 2.0 if v & (1 << 4) else 1.3 if v else 1.0
 
@@ -259,12 +310,13 @@ def build(jobs: int = 8, incremental: bool = False, arch: str = "gfx950", verbos
     BUILD.mkdir(exist_ok=True)
     gen = generate_catalog(BUILD)
     generate_embedded(BUILD)
+    generate_batch_cases(BUILD)
     hand = [HERE / "rk_runtime.cpp", HERE / "rk_fft_plan.cpp", HERE / "rk_jit.cpp", HERE / "rk_copy.hip",
-            HERE / "rk_batch.cpp"]
-    hand = [p for p in hand if p.exists()]
-    deps = [HERE / "fft_args.h", HERE / "fft_device.hpp", HERE / "rk_internal.hpp", REPO / "include" / "reikna_amd.h",
-            BUILD / "rk_embedded.inc"]
-    sources = hand + gen
+            HERE / "rk_batch.cpp", HERE / "rk_batch_kernel.hip"]
+    deps = [HERE / "fft_args.h", HERE / "fft_device.hpp", HERE / "fft_batch.h", HERE / "rk_internal.hpp",
+            REPO / "include" / "reikna_amd.h", BUILD / "rk_embedded.inc", BUILD / "batch_cases.inc"]
+    # the batch kernel is the longest compile: start it first
+    sources = [HERE / "rk_batch_kernel.hip"] + [p for p in hand if p.name != "rk_batch_kernel.hip"] + gen
     t0 = time.time()
     objs = []
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:

@@ -21,6 +21,7 @@
 typedef unsigned long long rk_u64;
 typedef long long rk_i64;
 typedef unsigned int rk_u32;
+typedef int rk_i32;
 
 // Compute mapping of threads to (instance c, thread-in-instance t):
 //   row mapping    (cmap = 0): c = tid / T, t = tid % T  (lanes along an instance)

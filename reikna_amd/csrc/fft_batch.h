@@ -1,0 +1,55 @@
+// fft_batch.h — descriptors of the persistent batch kernel (rk_fft_batch).
+//
+// A batch plan turns every transform of a batch into a chain of passes
+// (rk_pass_args, see fft_args.h) and every pass into tiles.  All tiles of all
+// passes form one queue, ordered level by level (every first pass, then every
+// second pass, ...).  Each work-group of the launch takes the next queue
+// position from an atomic counter as it starts; a tile of a pass that reads
+// the output of an earlier pass waits until all tiles of that pass are
+// counted done.  Every pass a tile can wait for sits earlier in the queue, so
+// its tiles were taken by work-groups that are already running, and every
+// wait ends (a poll limit turns a wait that does not end into an error flag).
+//
+// Shared by the host planner and device code, so it uses only builtin types.
+#ifndef RK_FFT_BATCH_H
+#define RK_FFT_BATCH_H
+
+#include "fft_args.h"
+
+#define RK_BATCH_THREADS 256
+
+#define RK_BATCH_BUF_IN 0u
+#define RK_BATCH_BUF_OUT 1u
+#define RK_BATCH_BUF_WS 2u
+
+struct rk_batch_pass {
+    rk_pass_args a;          // in/out are taken from the launch (src/dst below)
+    rk_u64 first;            // queue position of tile 0 of this pass
+    rk_u32 kcase;            // tile case of the persistent kernel
+    rk_u32 src, dst;         // RK_BATCH_BUF_*
+    rk_i32 wait;             // pass whose tiles must all be done first, -1: none
+    rk_u32 wait_tiles;       // tile count of that pass
+    rk_u32 signal;           // 1: finished tiles are counted (a later pass waits)
+    rk_u32 item_first;       // first pass of its transform (inverse: conjugate input)
+    rk_u32 item_last;        // last pass of its transform (inverse: conjugate output)
+    float fwd_scale;         // output scale of the forward transform
+    float inv_scale;         // output scale of the inverse transform
+    rk_u32 _pad;
+};
+
+struct rk_batch_work {
+    const rk_batch_pass* passes;
+    const rk_u32* tile_pass;  // pass of each queue position
+    rk_u32* done;             // finished tiles per pass (zero between launches)
+    rk_u32* state;            // [0] next queue position, [1] work-groups finished,
+                              // [2] set when a wait gave up after spin_limit polls
+    const float2* in;
+    float2* out;
+    float2* ws;
+    rk_u32 n_passes;
+    rk_u32 total;             // queue positions: tiles of all passes
+    rk_u32 spin_limit;
+    rk_u32 inverse;
+};
+
+#endif

@@ -353,6 +353,27 @@ struct PlainIO {
     }
 };
 
+// Plain strided IO whose loads and stores are coherent across the whole
+// device (relaxed agent-scope atomics: they bypass the caches that are not
+// shared by all CUs).  Used where one kernel launch hands data from one
+// work-group to another: after the producer's stores have completed, a
+// counter in memory is enough to publish them, with no cache write-back or
+// invalidation in between.
+struct CoherentIO {
+    static constexpr bool kBranchFree = true;
+    __device__ __forceinline__ float2 load(const rk_pass_args& a, rk_u64 /*o*/, rk_u64 /*m*/, rk_u64 /*i*/,
+                                           rk_i64 addr) const {
+        const rk_u64 bits = __hip_atomic_load(reinterpret_cast<rk_u64*>(const_cast<float2*>(a.in + addr)), __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+        return make_float2(__uint_as_float((unsigned)bits), __uint_as_float((unsigned)(bits >> 32)));
+    }
+    __device__ __forceinline__ void store(const rk_pass_args& a, rk_u64 /*o*/, rk_u64 /*k*/, rk_u64 /*i*/,
+                                          rk_i64 addr, float2 v) const {
+        const rk_u64 bits = (rk_u64)__float_as_uint(v.x) | ((rk_u64)__float_as_uint(v.y) << 32);
+        __hip_atomic_store(reinterpret_cast<rk_u64*>(a.out + addr), bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+};
+
 // Compile-time pass variant: which pre-/post-ops the pass applies and whether
 // it is the single-kernel Bluestein convolution.  Keeping them out of the
 // run-time arguments keeps plain passes free of table loads (and registers).
@@ -546,8 +567,14 @@ template <int L, int C> __host__ __device__ constexpr int tile_lds_bytes() {
     return C * lds_pitch<L>() * (tile_split<L, C>() ? 4 : 8);
 }
 
+// One tile (instances tile*C ... tile*C + C - 1) computed by threads
+// tid = 0 ... C*T - 1 with `lds` of at least tile_lds_bytes<L, C>() bytes.
+// Every value a thread computes and every address it writes is a function of
+// (a, tile, tid) alone, so a caller with more threads than C*T may give the
+// surplus threads tid = C*T - 1: they repeat that thread's work, writing the
+// same values to the same addresses, and take part in every barrier.
 template <int L, int C, int V, class IO>
-__device__ __forceinline__ void fft_pass_body(const rk_pass_args& a, const IO& io) {
+__device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& io, float2* lds, rk_u64 tile, int tid) {
     using P = Plan<L>;
     constexpr int T = P::T;
     constexpr int NT = C * T;
@@ -560,13 +587,8 @@ __device__ __forceinline__ void fft_pass_body(const rk_pass_args& a, const IO& i
     constexpr bool BLUE = Variant<V>::BLUE;
     constexpr bool TWID = Variant<V>::PRE == RK_VAR_PRE_TWIDDLE;
     constexpr bool SPLIT = tile_split<L, C>();
-    __shared__ float2 lds[SPLIT ? (C * LP + 1) / 2 : C * LP];
-
-    const rk_u64 tile = xcd_tile(a) + a.tile_base;
-    if (tile >= a.n_tiles) return;
     const rk_u64 inst0 = tile * (rk_u64)C;
 
-    const int tid = threadIdx.x;
     // compute mapping (see fft_args.h): lanes along an instance or across instances
     const bool colmap = a.cmap != 0;
     const int c = colmap ? (tid % C) : (tid / T);
@@ -683,6 +705,17 @@ __device__ __forceinline__ void fft_pass_body(const rk_pass_args& a, const IO& i
 
 // Threads per work-group of fft_pass_body<L, C, ...>.
 template <int L, int C> __host__ __device__ constexpr int pass_threads() { return C * Plan<L>::T; }
+
+// One work-group of a pass launch: the tile this work-group owns, with the
+// work-group's own LDS.
+template <int L, int C, int V, class IO>
+__device__ __forceinline__ void fft_pass_body(const rk_pass_args& a, const IO& io) {
+    constexpr int LP = lds_pitch<L>();
+    __shared__ float2 lds[tile_split<L, C>() ? (C * LP + 1) / 2 : C * LP];
+    const rk_u64 tile = xcd_tile(a) + a.tile_base;
+    if (tile >= a.n_tiles) return;
+    fft_pass_tile<L, C, V, IO>(a, io, lds, tile, (int)threadIdx.x);
+}
 
 }   // namespace rkfft
 

@@ -1,0 +1,141 @@
+// rk_batch_kernel.hip — the persistent kernel that runs a whole batch of
+// transforms of uneven lengths in one launch.
+//
+// No reference counterpart: reikna compiles and launches one computation per
+// shape (reikna/fft/fft.py:723-756).  The queue protocol is described in
+// fft_batch.h; the grid has one work-group per queue position.  All data
+// accesses of the kernel are device-coherent (CoherentIO), so a pass hands its
+// output to the next pass of the same transform through a tile counter alone:
+// agent-scope fences would write back or invalidate a whole XCD's L2 per tile.  (A
+// grid-stride loop over the queue was measured to lose per-case register
+// allocation: loop-invariant code of all cases is hoisted in front of the
+// loop, and 33 cases already spill.)  Every tile case is one fft_pass_tile<L, C, V> instantiation
+// (the same code as the one-pass-per-launch kernels) with C chosen so that a
+// tile needs at most RK_BATCH_THREADS threads; work-groups always have
+// RK_BATCH_THREADS threads and the surplus threads of a smaller tile repeat
+// the work of its last thread.  The case list comes from build.py
+// (batch_cases.inc): every length 2^k, 3*2^k and 5*2^k up to 4096 as a single
+// pass and as single-kernel Bluestein, and the pass lengths up to 256 of
+// multi-pass chains.
+#include "fft_device.hpp"
+#include "fft_batch.h"
+#include "batch_cases.inc"
+
+namespace {
+
+constexpr int kThreads = RK_BATCH_THREADS;
+
+#define RK_CASE_LDS(id, L, C, V) rkfft::tile_lds_bytes<L, C>(),
+#define RK_CASE_THREADS(id, L, C, V) rkfft::pass_threads<L, C>(),
+constexpr int kCaseLds[] = {RK_BATCH_CASES(RK_CASE_LDS)};
+constexpr int kCaseThreads[] = {RK_BATCH_CASES(RK_CASE_THREADS)};
+#undef RK_CASE_LDS
+#undef RK_CASE_THREADS
+
+constexpr int max_of(const int* v, int n) {
+    int m = 0;
+    for (int i = 0; i < n; ++i) m = v[i] > m ? v[i] : m;
+    return m;
+}
+constexpr int kNumCases = sizeof(kCaseLds) / sizeof(kCaseLds[0]);
+constexpr int kLdsBytes = max_of(kCaseLds, kNumCases);
+static_assert(max_of(kCaseThreads, kNumCases) <= kThreads, "a batch tile case needs more threads than a work-group has");
+static_assert(kLdsBytes <= 64 * 1024, "batch tile cases must leave room for two work-groups per CU");
+
+__device__ __forceinline__ void run_case(rk_u32 kc, const rk_pass_args& a, float2* lds, rk_u64 tile, int tid) {
+    switch (kc) {
+#define RK_CASE_RUN(id, L, C, V)                                                       \
+    case id: {                                                                         \
+        constexpr int nt = rkfft::pass_threads<L, C>();                                \
+        rkfft::fft_pass_tile<L, C, V>(a, rkfft::CoherentIO{}, lds, tile, tid < nt ? tid : nt - 1); \
+    } break;
+        RK_BATCH_CASES(RK_CASE_RUN)
+#undef RK_CASE_RUN
+        default:
+            break;
+    }
+}
+
+}  // namespace
+
+extern "C" __global__ void __launch_bounds__(kThreads) rk_fft_batch(rk_batch_work w) {
+    __shared__ float2 lds[(kLdsBytes + 7) / 8];
+    __shared__ rk_u32 s_item, s_last;
+    const int tid = threadIdx.x;
+    // one queue position per work-group, taken in the order work-groups start
+    if (tid == 0) s_item = __hip_atomic_fetch_add(&w.state[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const rk_u32 item = __builtin_amdgcn_readfirstlane(s_item);
+    if (item < w.total) {
+        const rk_u32 p = __builtin_amdgcn_readfirstlane(w.tile_pass[item]);
+        const rk_batch_pass* bp = w.passes + p;
+        const rk_i32 wait = bp->wait;
+        if (wait >= 0) {
+            // the producing pass's tiles were all taken earlier from the queue,
+            // by work-groups that are running
+            if (tid == 0) {
+                const rk_u32 need = bp->wait_tiles;
+                rk_u32 polls = 0;
+                while (__hip_atomic_load(&w.done[wait], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+                    if (++polls > w.spin_limit) {
+                        __hip_atomic_store(&w.state[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(4);
+                }
+            }
+            __syncthreads();
+        }
+        rk_pass_args a = bp->a;
+        const rk_u32 src = bp->src, dst = bp->dst;
+        a.in = src == RK_BATCH_BUF_IN ? w.in : src == RK_BATCH_BUF_OUT ? w.out : w.ws;
+        a.out = dst == RK_BATCH_BUF_OUT ? w.out : w.ws;
+        const bool inv = w.inverse != 0;
+        a.in_sign = (inv && bp->item_first) ? -1.0f : 1.0f;
+        a.out_sign = (inv && bp->item_last) ? -1.0f : 1.0f;
+        a.out_scale = inv ? bp->inv_scale : bp->fwd_scale;
+        run_case(bp->kcase, a, lds, (rk_u64)(item - bp->first), tid);
+        if (bp->signal) {
+            // every store of the tile has completed (they are device-coherent)
+            // before the tile is counted
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (tid == 0) __hip_atomic_fetch_add(&w.done[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    // the last work-group to finish clears the queue for the next launch
+    if (tid == 0) {
+        const rk_u32 prev = __hip_atomic_fetch_add(&w.state[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (prev + 1 == gridDim.x) ? 1u : 0u;
+    }
+    __syncthreads();
+    if (s_last) {
+        for (rk_u32 i = tid; i < w.n_passes; i += kThreads) w.done[i] = 0u;
+        if (tid == 0) {
+            w.state[0] = 0u;
+            w.state[1] = 0u;
+        }
+    }
+}
+
+// Host view of the kernel and its case list.
+#include "rk_internal.hpp"
+
+namespace rk {
+
+const void* batch_kernel() { return (const void*)&rk_fft_batch; }
+
+const std::vector<BatchCase>& batch_cases() {
+    static const std::vector<BatchCase> all = [] {
+        std::vector<BatchCase> v;
+#define RK_CASE_ENTRY(id, L, C, V) v.push_back(BatchCase{id, L, C, V, rkfft::pass_threads<L, C>(), rkfft::tile_lds_bytes<L, C>()});
+        RK_BATCH_CASES(RK_CASE_ENTRY)
+#undef RK_CASE_ENTRY
+        return v;
+    }();
+    return all;
+}
+
+int batch_lds_bytes() { return kLdsBytes; }
+
+}  // namespace rk

@@ -54,11 +54,14 @@ static int64_t factor_max() {
     return v;
 }
 
-// Split n into exactly p smooth factors <= fmax, the first as close to
-// n^(1/p) as possible (balanced factors keep every tile small).
-static bool split_into(int64_t n, int p, int64_t fmax, std::vector<int64_t>& out) {
+static bool length_ok(const ChainLimits& lim, int64_t n) { return lim.length_ok ? lim.length_ok(n) : is_smooth(n); }
+
+// Split n into exactly p factors <= lim.factor_max that lim accepts, the
+// first as close to n^(1/p) as possible (balanced factors keep every tile small).
+static bool split_into(int64_t n, int p, const ChainLimits& lim, std::vector<int64_t>& out) {
+    const int64_t fmax = lim.factor_max;
     if (p == 1) {
-        if (n <= fmax && is_smooth(n)) {
+        if (n <= fmax && length_ok(lim, n)) {
             out.push_back(n);
             return true;
         }
@@ -67,20 +70,38 @@ static bool split_into(int64_t n, int p, int64_t fmax, std::vector<int64_t>& out
     const double target = std::pow((double)n, 1.0 / p);
     std::vector<int64_t> cands;
     for (int64_t d = 2; d <= fmax && d <= n; ++d)
-        if (n % d == 0 && is_smooth(d)) cands.push_back(d);
+        if (n % d == 0 && length_ok(lim, d)) cands.push_back(d);
     std::sort(cands.begin(), cands.end(), [&](int64_t a, int64_t b) {
         return std::fabs(std::log((double)a / target)) < std::fabs(std::log((double)b / target));
     });
     for (int64_t d : cands) {
         const size_t keep = out.size();
         out.push_back(d);
-        if (split_into(n / d, p - 1, fmax, out)) return true;
+        if (split_into(n / d, p - 1, lim, out)) return true;
         out.resize(keep);
     }
     return false;
 }
 
-enum BufId { BUF_IN = 0, BUF_OUT = 1, BUF_WS0 = 2, BUF_WS1 = 3 };
+ChainLimits default_limits() {
+    ChainLimits l;
+    l.single_max = single_max();
+    l.factor_max = factor_max();
+    l.pow2_bluestein = false;
+    l.blue_single_max = l.single_max;
+    l.length_ok = nullptr;
+    return l;
+}
+
+int64_t TableBuilder::add(const std::string& key, const void* data, size_t n) {
+    auto it = index.find(key);
+    if (it != index.end()) return it->second;
+    size_t off = (bytes.size() + 255) & ~size_t(255);
+    bytes.resize(off + n);
+    memcpy(bytes.data() + off, data, n);
+    index[key] = (int64_t)off;
+    return (int64_t)off;
+}
 
 bool is_smooth(int64_t n) {
     if (n < 1) return false;
@@ -173,14 +194,7 @@ static void fft_host(std::vector<cd>& x) {
 // ---------------------------------------------------------------------------
 // plan structures
 // ---------------------------------------------------------------------------
-struct Layout {
-    int ndim;
-    int64_t shape[RK_MAX_DIMS];
-    int64_t strides[RK_MAX_DIMS];
-    int64_t offset;
-};
-
-static Layout contiguous_layout(int ndim, const int64_t* shape) {
+Layout contiguous_layout(int ndim, const int64_t* shape) {
     Layout l;
     l.ndim = ndim;
     l.offset = 0;
@@ -192,41 +206,6 @@ static Layout contiguous_layout(int ndim, const int64_t* shape) {
     }
     return l;
 }
-
-struct HostLoop {   // batch dims that could not be folded into the kernel's O / I
-    int64_t n, in_str, out_str;
-};
-
-struct PassPlan {
-    rk_pass_args args;
-    int src, dst;
-    int L, C, variant, threads, lds_bytes;
-    const void* aot = nullptr;      // ahead-of-time kernel
-    hipFunction_t jit = nullptr;    // run-time compiled kernel
-    bool user_kernel = false;       // replaced with rk_fft_plan_set_pass_kernel
-    bool is_first = false, is_last = false;
-    uint64_t grid = 0;
-    int axis = 0;
-    int kind = 0;
-    int64_t twl_off = -1, twhi_off = -1, twlo_off = -1, chirp_off = -1, bspec_off = -1;
-    std::vector<HostLoop> loops;
-    uint64_t inst_per_outer = 0;   // H * NS * I: instances per outer-batch index
-    int64_t outer_count = 1;       // O
-};
-
-struct TableBuilder {
-    std::vector<unsigned char> bytes;
-    std::map<std::string, int64_t> index;
-    int64_t add(const std::string& key, const void* data, size_t n) {
-        auto it = index.find(key);
-        if (it != index.end()) return it->second;
-        size_t off = (bytes.size() + 255) & ~size_t(255);
-        bytes.resize(off + n);
-        memcpy(bytes.data() + off, data, n);
-        index[key] = (int64_t)off;
-        return (int64_t)off;
-    }
-};
 
 }  // namespace rk
 
@@ -256,11 +235,12 @@ namespace rk {
 // planning helpers
 // ---------------------------------------------------------------------------
 
-// Split a smooth n into pass lengths, each <= factor_max(), as balanced as possible.
-static bool split_factors(int64_t n, std::vector<int64_t>& out) {
+// Split a smooth n into pass lengths, each <= lim.factor_max and accepted by
+// lim.length_ok, as balanced as possible.
+static bool split_factors(int64_t n, const ChainLimits& lim, std::vector<int64_t>& out) {
     out.clear();
-    const int64_t fmax = factor_max();
-    if (n <= fmax) {
+    const int64_t fmax = lim.factor_max;
+    if (n <= fmax && length_ok(lim, n)) {
         out.push_back(n);
         return true;
     }
@@ -269,16 +249,17 @@ static bool split_factors(int64_t n, std::vector<int64_t>& out) {
         double cap = 1.0;
         for (int k = 0; k < p; ++k) cap *= (double)fmax;
         if (cap < (double)n) continue;
-        if (split_into(n, p, fmax, out)) return true;
+        if (split_into(n, p, lim, out)) return true;
         out.clear();
     }
     return false;
 }
 
-// Smallest M >= 2n-1 of the form 2^k, 3*2^k or 5*2^k.
-static int64_t bluestein_length(int64_t n) {
+// Smallest M >= 2n-1 of the form 2^k, 3*2^k or 5*2^k (2^k only with pow2).
+static int64_t bluestein_length(int64_t n, bool pow2 = false) {
     int64_t need = 2 * n - 1, best = 0;
     for (int64_t base : {1, 3, 5}) {
+        if (pow2 && base != 1) continue;
         int64_t m = base;
         while (m < need) m *= 2;
         if (!best || m < best) best = m;
@@ -286,19 +267,10 @@ static int64_t bluestein_length(int64_t n) {
     return best;
 }
 
-struct AxisPass {
-    int L;
-    int64_t Nax;      // axis length the Stockham step works on (N or M)
-    int64_t NS, H;
-    bool blue_single;
-    uint32_t pre_ops, post_ops;
-    int64_t n_valid_in, n_valid_out;
-    int64_t in_len, out_len;   // axis length in the src / dst layouts
-};
-
 // The passes of one axis of length n.
-static int axis_passes(int64_t n, uint32_t flags, std::vector<AxisPass>& out) {
+int axis_passes(int64_t n, uint32_t flags, const ChainLimits& lim, std::vector<AxisPass>& out) {
     out.clear();
+    const int64_t kSingle = lim.single_max;
     const bool smooth = is_smooth(n) && !(flags & RK_FFT_FORCE_BLUESTEIN);
     auto stockham_chain = [&](int64_t N, const std::vector<int64_t>& f, int64_t in_len, int64_t out_len,
                               uint32_t first_pre, uint32_t last_post, int64_t valid_in, int64_t valid_out) {
@@ -320,13 +292,10 @@ static int axis_passes(int64_t n, uint32_t flags, std::vector<AxisPass>& out) {
             ns *= f[p];
         }
     };
-    if (smooth) {
-        std::vector<int64_t> f;
-        if (n <= kSingleMax && !(flags & RK_FFT_FORCE_MULTIPASS)) {
-            f.push_back(n);
-        } else if (!split_factors(n, f)) {
-            return fail(RK_ERR_UNSUPPORTED, "cannot split axis length %lld into passes", (long long)n);
-        }
+    std::vector<int64_t> f;
+    const bool single_ok = n <= kSingle && !(flags & RK_FFT_FORCE_MULTIPASS) && length_ok(lim, n);
+    if (smooth && (single_ok || split_factors(n, lim, f))) {
+        if (single_ok) f.assign(1, n);
         if ((flags & RK_FFT_FORCE_MULTIPASS) && f.size() == 1 && n > 1) {
             // split a single-pass length into two passes for testing the chain
             int64_t d = 0;
@@ -337,8 +306,11 @@ static int axis_passes(int64_t n, uint32_t flags, std::vector<AxisPass>& out) {
         stockham_chain(n, f, n, n, 0u, 0u, n, n);
         return RK_OK;
     }
-    const int64_t m = bluestein_length(n);
-    if (m <= kSingleMax && !(flags & RK_FFT_FORCE_MULTIPASS)) {
+    if (smooth && !lim.length_ok)
+        return fail(RK_ERR_UNSUPPORTED, "cannot split axis length %lld into passes", (long long)n);
+    const int64_t m = bluestein_length(n, lim.pow2_bluestein);
+    const bool multi = m > lim.blue_single_max || (flags & RK_FFT_FORCE_MULTIPASS);
+    if (!multi) {
         AxisPass ap;
         ap.L = (int)m;
         ap.Nax = m;
@@ -354,8 +326,8 @@ static int axis_passes(int64_t n, uint32_t flags, std::vector<AxisPass>& out) {
         out.push_back(ap);
         return RK_OK;
     }
-    std::vector<int64_t> f;
-    if (!split_factors(m, f)) return fail(RK_ERR_UNSUPPORTED, "cannot split Bluestein length %lld", (long long)m);
+    if (!split_factors(m, lim, f))
+        return fail(RK_ERR_UNSUPPORTED, "cannot split Bluestein length %lld", (long long)m);
     if (f.size() == 1) {   // forced multi-pass of a short M
         int64_t d = 0;
         for (int64_t k = 2; k * k <= m; ++k)
@@ -418,6 +390,87 @@ static void fold_batch(const Layout& src, const Layout& dst, int axis, int64_t& 
     for (auto& g : inner) loops.push_back(g);
 }
 
+int describe_pass(const AxisPass& ap, const Layout& ls, const Layout& ld, int axis, TileWidthFn tile_width,
+                  PassPlan& pp) {
+    memset(&pp.args, 0, sizeof(pp.args));
+    pp.axis = axis;
+    pp.L = ap.L;
+    pp.kind = ap.blue_single ? 1 : 0;
+    if (ap.blue_single) {
+        pp.variant = VAR_BLUE;
+    } else {
+        int pre = (ap.pre_ops & RK_PRE_TWIDDLE) ? 1 : (ap.pre_ops & RK_PRE_CHIRP) ? 2 : 0;
+        int post = (ap.post_ops & RK_POST_KMUL_CONJ) ? 1 : (ap.post_ops & RK_POST_CHIRP) ? 2 : 0;
+        if ((ap.pre_ops & RK_PRE_TWIDDLE) && (ap.pre_ops & RK_PRE_CHIRP))
+            return fail(RK_ERR_UNSUPPORTED, "internal: pass with both chirp and twiddle pre-ops");
+        pp.variant = pre | (post << 2);
+    }
+    rk_pass_args& a = pp.args;
+    int64_t O, so_in, so_out, I, si_in, si_out;
+    fold_batch(ls, ld, axis, O, so_in, so_out, I, si_in, si_out, pp.loops);
+    a.dim_I = (rk_u64)I;
+    a.dim_NS = (rk_u64)ap.NS;
+    a.dim_H = (rk_u64)ap.H;
+    a.n_inst = (rk_u64)(O * ap.H * ap.NS * I);
+    a.in_off = ls.offset;
+    a.out_off = ld.offset;
+    a.in_so = so_in;
+    a.out_so = so_out;
+    a.in_si = si_in;
+    a.out_si = si_out;
+    a.in_sa = ls.strides[axis];
+    a.out_sa = ld.strides[axis];
+    a.N = (rk_u64)ap.Nax;
+    a.pos_mult = (rk_u64)(ap.Nax / ap.L);
+    a.tw_mult = (rk_u64)(ap.Nax / (ap.NS * ap.L));
+    a.pre_ops = ap.pre_ops;
+    a.post_ops = ap.post_ops;
+    a.n_valid_in = (rk_u64)ap.n_valid_in;
+    a.n_valid_out = (rk_u64)ap.n_valid_out;
+    a.in_sign = 1.0f;
+    a.out_sign = 1.0f;
+    a.out_scale = 1.0f;
+
+    // global-memory walk of each side
+    const int64_t L = ap.L;
+    const int64_t pos_in = (int64_t)a.pos_mult * a.in_sa;
+    const int64_t inst_in = I > 1 ? si_in : (ap.NS > 1 || ap.H > 1) ? a.in_sa : so_in;
+    const bool flat_in = ap.H == 1 && ap.NS == 1 && I == 1 && a.in_sa == 1 && (O == 1 || so_in == L) &&
+                         ap.pre_ops == 0 && ap.n_valid_in == ap.Nax && pp.loops.empty();
+    const int64_t pos_out = ap.NS * a.out_sa;
+    const int64_t inst_out = I > 1 ? si_out : ap.NS > 1 ? a.out_sa : ap.H > 1 ? L * a.out_sa : so_out;
+    const bool flat_out = ap.NS == 1 && I == 1 && a.out_sa == 1 &&
+                          (ap.H == 1 ? (O == 1 || so_out == L) : (O == 1 || so_out == ap.Nax)) &&
+                          ap.post_ops == 0 && ap.n_valid_out == ap.Nax && pp.loops.empty();
+    // A side is column-like when adjacent instances are adjacent in
+    // memory and its elements are not; the compute mapping follows
+    // the column-like side(s) so they are read/written straight from
+    // registers, and a row-like side under that mapping is staged
+    // through LDS.
+    const bool col_in = pos_in != 1 && inst_in == 1;
+    const bool col_out = pos_out != 1 && inst_out == 1;
+    const int T = threads_per_instance(ap.L);
+    a.cmap = (col_in || col_out) ? 1u : 0u;
+    auto side_mode = [&](bool col, bool flat, int64_t pos) -> rk_u32 {
+        if (a.cmap) {
+            if (col) return RK_MODE_DIRECT;
+            if (flat) return RK_MODE_FLAT;
+            return pos == 1 ? RK_MODE_ROW : RK_MODE_DIRECT;
+        }
+        if (flat && T < 32) return RK_MODE_FLAT;   // short rows: stream the tile linearly
+        return RK_MODE_DIRECT;
+    };
+    a.load_mode = side_mode(col_in, flat_in, pos_in);
+    a.store_mode = side_mode(col_out, flat_out, pos_out);
+    pp.C = tile_width(ap.L, a.cmap != 0);
+    a.n_tiles = (a.n_inst + (rk_u64)pp.C - 1) / (rk_u64)pp.C;
+    pp.inst_per_outer = (uint64_t)(ap.H * ap.NS * I);
+    pp.outer_count = O;
+    return RK_OK;
+}
+
+static int plan_tile_width(int L, bool colmap) { return colmap ? tile_c_col(L) : tile_c_row(L); }
+
 static int find_kernel(PassPlan& pp, uint32_t flags) {
     for (const KernelEntry& e : catalog())
         if (e.L == pp.L && e.C == pp.C && e.variant == pp.variant) {
@@ -440,7 +493,7 @@ static int find_kernel(PassPlan& pp, uint32_t flags) {
     return RK_OK;
 }
 
-static int build_tables(rk_fft_plan* plan, TableBuilder& tb, PassPlan& pp, const AxisPass& ap, int64_t nreal) {
+int build_tables(TableBuilder& tb, PassPlan& pp, const AxisPass& ap, int64_t nreal) {
     // in-pass stage twiddles W_L^e
     {
         std::vector<float2> w(pp.L);
@@ -503,7 +556,6 @@ static int build_tables(rk_fft_plan* plan, TableBuilder& tb, PassPlan& pp, const
         pp.bspec_off = tb.add("bspec" + std::to_string(nreal) + "_" + std::to_string(m), bs.data(),
                               bs.size() * sizeof(float2));
     }
-    (void)plan;
     return RK_OK;
 }
 
@@ -580,7 +632,7 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
         const int axis = work[wi];
         const int64_t n = d->shape[axis];
         std::vector<AxisPass> aps;
-        int rc = axis_passes(n, d->flags, aps);
+        int rc = axis_passes(n, d->flags, default_limits(), aps);
         if (rc != RK_OK) return rc;
         if (aps.size() >= 3) two_ws = true;
         int ws_next = BUF_WS0;
@@ -610,89 +662,17 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
             }
 
             PassPlan pp;
-            memset(&pp.args, 0, sizeof(pp.args));
             pp.src = cur;
             pp.dst = dst;
-            pp.axis = axis;
-            pp.L = ap.L;
-            pp.kind = ap.blue_single ? 1 : 0;
-            if (ap.blue_single) {
-                pp.variant = VAR_BLUE;
-            } else {
-                int pre = (ap.pre_ops & RK_PRE_TWIDDLE) ? 1 : (ap.pre_ops & RK_PRE_CHIRP) ? 2 : 0;
-                int post = (ap.post_ops & RK_POST_KMUL_CONJ) ? 1 : (ap.post_ops & RK_POST_CHIRP) ? 2 : 0;
-                if ((ap.pre_ops & RK_PRE_TWIDDLE) && (ap.pre_ops & RK_PRE_CHIRP))
-                    return fail(RK_ERR_UNSUPPORTED, "internal: pass with both chirp and twiddle pre-ops");
-                pp.variant = pre | (post << 2);
-            }
-            rk_pass_args& a = pp.args;
-            int64_t O, so_in, so_out, I, si_in, si_out;
-            fold_batch(ls, ld, axis, O, so_in, so_out, I, si_in, si_out, pp.loops);
-            a.dim_I = (rk_u64)I;
-            a.dim_NS = (rk_u64)ap.NS;
-            a.dim_H = (rk_u64)ap.H;
-            a.n_inst = (rk_u64)(O * ap.H * ap.NS * I);
-            a.in_off = ls.offset;
-            a.out_off = ld.offset;
-            a.in_so = so_in;
-            a.out_so = so_out;
-            a.in_si = si_in;
-            a.out_si = si_out;
-            a.in_sa = ls.strides[axis];
-            a.out_sa = ld.strides[axis];
-            a.N = (rk_u64)ap.Nax;
-            a.pos_mult = (rk_u64)(ap.Nax / ap.L);
-            a.tw_mult = (rk_u64)(ap.Nax / (ap.NS * ap.L));
-            a.pre_ops = ap.pre_ops;
-            a.post_ops = ap.post_ops;
-            a.n_valid_in = (rk_u64)ap.n_valid_in;
-            a.n_valid_out = (rk_u64)ap.n_valid_out;
-            a.in_sign = 1.0f;
-            a.out_sign = 1.0f;
-            a.out_scale = 1.0f;
-
-            // global-memory walk of each side
-            const int64_t L = ap.L;
-            const int64_t pos_in = (int64_t)a.pos_mult * a.in_sa;
-            const int64_t inst_in = I > 1 ? si_in : (ap.NS > 1 || ap.H > 1) ? a.in_sa : so_in;
-            const bool flat_in = ap.H == 1 && ap.NS == 1 && I == 1 && a.in_sa == 1 && (O == 1 || so_in == L) &&
-                                 ap.pre_ops == 0 && ap.n_valid_in == ap.Nax && pp.loops.empty();
-            const int64_t pos_out = ap.NS * a.out_sa;
-            const int64_t inst_out = I > 1 ? si_out : ap.NS > 1 ? a.out_sa : ap.H > 1 ? L * a.out_sa : so_out;
-            const bool flat_out = ap.NS == 1 && I == 1 && a.out_sa == 1 &&
-                                  (ap.H == 1 ? (O == 1 || so_out == L) : (O == 1 || so_out == ap.Nax)) &&
-                                  ap.post_ops == 0 && ap.n_valid_out == ap.Nax && pp.loops.empty();
-            // A side is column-like when adjacent instances are adjacent in
-            // memory and its elements are not; the compute mapping follows
-            // the column-like side(s) so they are read/written straight from
-            // registers, and a row-like side under that mapping is staged
-            // through LDS.
-            const bool col_in = pos_in != 1 && inst_in == 1;
-            const bool col_out = pos_out != 1 && inst_out == 1;
-            const int T = threads_per_instance(ap.L);
-            a.cmap = (col_in || col_out) ? 1u : 0u;
-            auto side_mode = [&](bool col, bool flat, int64_t pos) -> rk_u32 {
-                if (a.cmap) {
-                    if (col) return RK_MODE_DIRECT;
-                    if (flat) return RK_MODE_FLAT;
-                    return pos == 1 ? RK_MODE_ROW : RK_MODE_DIRECT;
-                }
-                if (flat && T < 32) return RK_MODE_FLAT;   // short rows: stream the tile linearly
-                return RK_MODE_DIRECT;
-            };
-            a.load_mode = side_mode(col_in, flat_in, pos_in);
-            a.store_mode = side_mode(col_out, flat_out, pos_out);
-            pp.C = a.cmap ? tile_c_col(ap.L) : tile_c_row(ap.L);
+            rc = describe_pass(ap, ls, ld, axis, plan_tile_width, pp);
+            if (rc != RK_OK) return rc;
 
             rc = find_kernel(pp, d->flags);
             if (rc != RK_OK) return rc;
-            a.n_tiles = (a.n_inst + (rk_u64)pp.C - 1) / (rk_u64)pp.C;
             const uint64_t max_grid = (uint64_t(1) << 31) / (uint64_t)pp.threads;
-            pp.grid = std::min<uint64_t>(a.n_tiles, max_grid);
-            rc = build_tables(plan.get(), tb, pp, ap, n);
+            pp.grid = std::min<uint64_t>(pp.args.n_tiles, max_grid);
+            rc = build_tables(tb, pp, ap, n);
             if (rc != RK_OK) return rc;
-            pp.inst_per_outer = (uint64_t)(ap.H * ap.NS * I);
-            pp.outer_count = O;
             plan->passes.push_back(pp);
             cur = dst;
             cur_shape[axis] = ap.out_len;

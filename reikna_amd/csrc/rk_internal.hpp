@@ -6,6 +6,7 @@
 
 #include <cstdarg>
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -59,6 +60,92 @@ bool is_smooth(int64_t n);
 // A pass kernel from the run-time compiler: (L, C, variant) instantiated with
 // the plain strided IO policy; cached per process and on disk.
 int jit_pass_kernel(int L, int C, int variant, hipFunction_t* fn, int* threads, int* lds_bytes);
+
+// ---------------------------------------------------------------------------
+// planner pieces shared by single-shape plans and batch plans
+// ---------------------------------------------------------------------------
+struct Layout {   // an N-d strided view (element strides / offset)
+    int ndim;
+    int64_t shape[RK_MAX_DIMS];
+    int64_t strides[RK_MAX_DIMS];
+    int64_t offset;
+};
+Layout contiguous_layout(int ndim, const int64_t* shape);
+
+struct HostLoop {   // batch dims that could not be folded into the kernel's O / I
+    int64_t n, in_str, out_str;
+};
+
+enum BufId { BUF_IN = 0, BUF_OUT = 1, BUF_WS0 = 2, BUF_WS1 = 3 };
+
+// One pass of an axis: a radix-L Stockham step over an axis of length Nax.
+struct AxisPass {
+    int L;
+    int64_t Nax;      // axis length the Stockham step works on (N or M)
+    int64_t NS, H;
+    bool blue_single;
+    uint32_t pre_ops, post_ops;
+    int64_t n_valid_in, n_valid_out;
+    int64_t in_len, out_len;   // axis length in the src / dst layouts
+};
+
+// Limits of the pass chain of one axis.
+struct ChainLimits {
+    int64_t single_max;    // longest single-pass DFT
+    int64_t factor_max;    // longest pass of a multi-pass chain
+    int64_t blue_single_max;   // longest single-kernel Bluestein length M
+    bool pow2_bluestein;   // Bluestein lengths are powers of two
+    bool (*length_ok)(int64_t L);   // pass lengths a kernel exists for (null: any smooth length)
+};
+ChainLimits default_limits();
+
+// The passes of one axis of length n.
+int axis_passes(int64_t n, uint32_t flags, const ChainLimits& lim, std::vector<AxisPass>& out);
+
+struct PassPlan {
+    rk_pass_args args;
+    int src, dst;
+    int L, C, variant, threads, lds_bytes;
+    const void* aot = nullptr;      // ahead-of-time kernel
+    hipFunction_t jit = nullptr;    // run-time compiled kernel
+    bool user_kernel = false;       // replaced with rk_fft_plan_set_pass_kernel
+    bool is_first = false, is_last = false;
+    uint64_t grid = 0;
+    int axis = 0;
+    int kind = 0;
+    int64_t twl_off = -1, twhi_off = -1, twlo_off = -1, chirp_off = -1, bspec_off = -1;
+    std::vector<HostLoop> loops;
+    uint64_t inst_per_outer = 0;   // H * NS * I: instances per outer-batch index
+    int64_t outer_count = 1;       // O
+};
+
+// Device tables of a plan, appended once per key and 256-byte aligned.
+struct TableBuilder {
+    std::vector<unsigned char> bytes;
+    std::map<std::string, int64_t> index;
+    int64_t add(const std::string& key, const void* data, size_t n);
+};
+
+// Tile width C for a pass of length L under the row (colmap = false) or
+// column compute mapping.
+typedef int (*TileWidthFn)(int L, bool colmap);
+
+// Fills pp.args (dims, strides, walk modes, ops), pp.variant, pp.C and pp.loops
+// for pass `ap` from layout ls to ld along `axis`.
+int describe_pass(const AxisPass& ap, const Layout& ls, const Layout& ld, int axis, TileWidthFn tile_width,
+                  PassPlan& pp);
+
+// Stage twiddles, inter-pass twiddles, chirp and Bluestein spectrum of a pass
+// (nreal: the transform length before Bluestein padding).
+int build_tables(TableBuilder& tb, PassPlan& pp, const AxisPass& ap, int64_t nreal);
+
+// Tile cases of the persistent batch kernel (rk_batch_kernel.hip).
+struct BatchCase {
+    int id, L, C, variant, threads, lds_bytes;
+};
+const std::vector<BatchCase>& batch_cases();
+const void* batch_kernel();
+int batch_lds_bytes();
 
 // Strided N-d copy (trivial plans: no axis longer than 1).
 struct CopyArgs {

@@ -1,0 +1,214 @@
+"""Batches of 1-D complex64 transforms of uneven lengths.
+
+No reference counterpart: reikna builds one computation per array shape
+(reikna/fft/fft.py:630-756), so a batch of transforms of different lengths
+costs one compiled computation and one chain of launches per length.  Here a
+:class:`BatchFFT` plans every transform of the batch as a chain of passes and
+runs all of them in one launch of the batch kernel (``mode="queue"``); the
+per-item path (``mode="grouped"``: one single-shape plan per item, launched
+one after the other) is kept for comparison.
+
+Transforms follow numpy.fft conventions: forward ``X[k] = sum_n x[n]
+exp(-2 pi i k n / N)``, inverse with ``+`` and a factor 1/N (or the factor on
+the forward transform for items created with ``scale=True``).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from collections.abc import Sequence
+from dataclasses import dataclass
+
+from .. import _lib
+from ..runtime import Array, Buffer, Device, Queue, VirtualDevice
+
+__all__ = ["BatchFFT", "BatchItem", "BatchOptions"]
+
+
+@dataclass(frozen=True)
+class BatchItem:
+    """``count`` transforms of ``length`` complex64 elements; transform j of
+    the item reads ``in[in_offset + j * in_dist + n]`` and writes
+    ``out[out_offset + j * out_dist + k]`` (offsets and distances in
+    elements)."""
+
+    length: int
+    count: int = 1
+    in_offset: int = 0
+    out_offset: int = 0
+    in_dist: int = 0
+    out_dist: int = 0
+    scale: bool = False   # forward result multiplied by 1/N (inverse unscaled)
+
+
+@dataclass(frozen=True)
+class BatchOptions:
+    """Planner knobs (the autotuner searches over them; 0 = library default)."""
+
+    mode: str = "queue"      # "queue" (one launch) or "grouped" (one plan per item)
+    single_max: int = 0      # longest single-pass length (<= 4096)
+    factor_max: int = 0      # longest pass of a multi-pass chain (<= 256)
+    blue_max: int = 0        # longest single-kernel Bluestein length (<= 2048)
+    order: int = 0           # queue order in a level: 0 largest tiles first, 1 item order
+
+    def to_c(self, dry: bool = False) -> _lib.BatchOptions:
+        if self.mode not in ("queue", "grouped"):
+            raise ValueError(f"unknown batch mode {self.mode!r}")
+        o = _lib.BatchOptions()
+        o.mode = _lib.RK_BATCH_MODE_QUEUE if self.mode == "queue" else _lib.RK_BATCH_MODE_GROUPED
+        o.flags = _lib.RK_BATCH_DRY_RUN if dry else 0
+        o.single_max = int(self.single_max)
+        o.factor_max = int(self.factor_max)
+        o.blue_max = int(self.blue_max)
+        o.order = int(self.order)
+        return o
+
+
+def packed_items(lengths: Sequence[int], counts: Sequence[int] | None = None, scale: bool = False) -> list[BatchItem]:
+    """Items laid out back to back in one buffer (in and out alike)."""
+    counts = list(counts) if counts is not None else [1] * len(lengths)
+    if len(counts) != len(lengths):
+        raise ValueError("lengths and counts differ in size")
+    items, off = [], 0
+    for n, c in zip(lengths, counts):
+        n, c = int(n), int(c)
+        items.append(BatchItem(n, c, off, off, n, n, scale))
+        off += n * c
+    return items
+
+
+class BatchFFT:
+    """A compiled batch of uneven 1-D complex64 transforms on one device."""
+
+    def __init__(self, device: Device | VirtualDevice, items: Sequence[BatchItem], options: BatchOptions | None = None):
+        self.device = device
+        self.items = list(items)
+        self.options = options or BatchOptions()
+        self.virtual = isinstance(device, VirtualDevice)
+        arr = (_lib.BatchItem * max(1, len(self.items)))()
+        for k, it in enumerate(self.items):
+            arr[k].length = int(it.length)
+            arr[k].count = int(it.count)
+            arr[k].in_offset = int(it.in_offset)
+            arr[k].out_offset = int(it.out_offset)
+            arr[k].in_dist = int(it.in_dist)
+            arr[k].out_dist = int(it.out_dist)
+            arr[k].flags = _lib.RK_BATCH_SCALE if it.scale else 0
+        if not self.virtual:
+            device.activate()
+        opts = self.options.to_c(dry=self.virtual)
+        h = ctypes.c_void_p()
+        _lib.call("rk_batch_plan_create_ex", arr, len(self.items), ctypes.byref(opts), ctypes.byref(h))
+        self._h = h.value
+        sz = ctypes.c_size_t(0)
+        _lib.call("rk_batch_plan_workspace_size", self._h, ctypes.byref(sz))
+        self.workspace_bytes = int(sz.value)
+        self._ws: Buffer | None = None
+        # elements each side must hold
+        self.in_elements = max((it.in_offset + (it.count - 1) * it.in_dist + it.length for it in self.items if it.count),
+                               default=0)
+        self.out_elements = max(
+            (it.out_offset + (it.count - 1) * it.out_dist + it.length for it in self.items if it.count), default=0
+        )
+
+    @classmethod
+    def packed(
+        cls,
+        device: Device | VirtualDevice,
+        lengths: Sequence[int],
+        counts: Sequence[int] | None = None,
+        scale: bool = False,
+        options: BatchOptions | None = None,
+    ) -> BatchFFT:
+        return cls(device, packed_items(lengths, counts, scale), options)
+
+    # -- introspection ----------------------------------------------------
+    @property
+    def info(self) -> dict:
+        i = _lib.BatchInfo()
+        _lib.call("rk_batch_plan_info", self._h, ctypes.byref(i))
+        return {f: int(getattr(i, f)) for f, _ in _lib.BatchInfo._fields_}
+
+    def check(self) -> None:
+        """Raises RuntimeError if a dependency wait of an earlier call gave up
+        (synchronises the device)."""
+        hit = ctypes.c_int(0)
+        _lib.call("rk_batch_plan_check", self._h, ctypes.byref(hit))
+        if hit.value:
+            raise RuntimeError("batch kernel: a dependency wait hit its poll limit; results are not valid")
+
+    # -- execution ----------------------------------------------------------
+    def _workspace(self) -> int:
+        if not self.workspace_bytes:
+            return 0
+        if self._ws is None:
+            self._ws = Buffer(self.device, self.workspace_bytes)
+        return self._ws.ptr
+
+    def __call__(self, queue: Queue, output: Array, input: Array, inverse: int = 0) -> None:
+        if self.virtual:
+            raise ValueError("a batch planned for a VirtualDevice cannot execute")
+        for name, arr, need in (("output", output, self.out_elements), ("input", input, self.in_elements)):
+            if arr.dtype.itemsize != 8 or arr.dtype.kind != "c":
+                raise ValueError(f"{name} must be complex64")
+            if arr.size < need:
+                raise ValueError(f"{name} holds {arr.size} elements, the batch needs {need}")
+        _lib.call(
+            "rk_batch_execute", self._h, ctypes.c_void_p(output.data_ptr), ctypes.c_void_p(input.data_ptr),
+            int(bool(inverse)), ctypes.c_void_p(self._workspace() or None), queue.stream,
+        )
+
+    def __del__(self) -> None:
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                _lib.load().rk_batch_plan_destroy(h)
+            except Exception:  # pragma: no cover - interpreter shutdown
+                pass
+            self._h = None
+
+
+def _is_prime(n: int) -> bool:
+    if n < 2:
+        return False
+    if n % 2 == 0:
+        return n == 2
+    f = 3
+    while f * f <= n:
+        if n % f == 0:
+            return False
+        f += 2
+    return True
+
+
+def sample_lengths(count: int, lo: int = 64, hi: int = 65536, seed: int = 5) -> list[int]:
+    """Transform lengths log-uniform in [lo, hi], each snapped to one of four
+    families drawn with equal probability: 2^k, 3*2^k, 5*2^k (nearest in log
+    scale) or the next prime (the mixed-length batch workload)."""
+    import numpy
+
+    rng = numpy.random.default_rng(seed)
+    out: list[int] = []
+    for _ in range(count):
+        x = float(numpy.exp(rng.uniform(numpy.log(lo), numpy.log(hi))))
+        fam = int(rng.integers(4))
+        if fam < 3:
+            base = (1, 3, 5)[fam]
+            k = max(0, round(numpy.log2(x / base)))
+            n = base * 2**k
+            while n > hi and k > 0:
+                k -= 1
+                n = base * 2**k
+            while n < lo:
+                k += 1
+                n = base * 2**k
+        else:
+            n = max(lo, int(x))
+            while not _is_prime(n):
+                n += 1
+            if n > hi:
+                n = int(x)
+                while not _is_prime(n):
+                    n -= 1
+        out.append(int(n))
+    return out

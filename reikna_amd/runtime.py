@@ -314,8 +314,57 @@ class Queue:
     def synchronize(self) -> None:
         _lib.call("rk_stream_synchronize", self._stream)
 
+    def capture(self) -> _Capture:
+        """Record the work enqueued inside a ``with`` block into a HIP graph
+        instead of running it::
+
+            with queue.capture() as cap:
+                fft(queue, out, inp)
+            cap.graph.launch(queue)    # replays the recorded launches
+        """
+        return _Capture(self)
+
     def __repr__(self) -> str:
         return f"Queue({self.device}, stream=0x{(self._stream or 0):x})"
+
+
+class Graph:
+    """An instantiated HIP graph (the launches recorded by Queue.capture)."""
+
+    def __init__(self, handle: int):
+        self._h = handle
+        weakref.finalize(self, _destroy_graph, handle)
+
+    def launch(self, queue: Queue) -> None:
+        _lib.call("rk_graph_launch", self._h, queue.stream)
+
+
+def _destroy_graph(h: int) -> None:
+    try:
+        _lib.load().rk_graph_destroy(h)
+    except Exception:  # pragma: no cover - interpreter shutdown
+        pass
+
+
+class _Capture:
+    def __init__(self, queue: Queue):
+        self.queue = queue
+        self.graph: Graph | None = None
+
+    def __enter__(self) -> _Capture:
+        if self.queue.stream is None:
+            raise ValueError("the default stream cannot be captured; use a created Queue")
+        _lib.call("rk_graph_begin", self.queue.stream)
+        return self
+
+    def __exit__(self, exc_type: Any, exc: Any, tb: Any) -> None:
+        h = ctypes.c_void_p()
+        status = _lib.load().rk_graph_end(self.queue.stream, ctypes.byref(h))
+        if exc_type is None:
+            _lib.check(status, "rk_graph_end")
+            self.graph = Graph(h.value)
+        elif h.value:
+            _destroy_graph(h.value)
 
 
 def _destroy_stream(stream: int) -> None:

@@ -1,0 +1,105 @@
+"""Batch planning without a GPU (dry-run plans on a VirtualDevice)."""
+
+import pytest
+
+from reikna_amd.fft import BatchFFT, BatchItem, BatchOptions
+from reikna_amd.fft.batch import packed_items, sample_lengths
+from reikna_amd.runtime import VirtualDevice
+
+
+@pytest.fixture(scope="module")
+def vdev():
+    return VirtualDevice()
+
+
+def info(vdev, lengths, counts=None, **opts):
+    return BatchFFT(vdev, packed_items(lengths, counts), BatchOptions(**opts)).info
+
+
+def test_single_pass_lengths_one_level(vdev):
+    i = info(vdev, [64, 96, 160, 1024, 3072, 4096])
+    assert i["levels"] == 1 and i["passes"] == 6 and i["launches"] == 1
+    assert i["workspace_bytes"] == 0
+
+
+def test_chain_lengths(vdev):
+    # 8192 -> two passes; 65521 (prime) -> Bluestein over 2^17: two chains of three passes
+    assert info(vdev, [8192])["levels"] == 2
+    assert info(vdev, [49152])["levels"] == 2          # 3 * 2^14 = 192 * 256
+    assert info(vdev, [65521])["levels"] == 6
+    assert info(vdev, [997])["levels"] == 1            # single-kernel Bluestein over 2048
+    assert info(vdev, [5003])["levels"] == 4           # Bluestein over 2^14 = 128 * 128, twice
+    i = info(vdev, [8192, 64, 65521])
+    assert i["levels"] == 6 and i["passes"] == 2 + 1 + 6 and i["launches"] == 1
+    assert i["workspace_bytes"] >= 8 * (8192 + 2 * 131072)   # one region for two passes, two for more
+
+
+def test_lengths_without_a_case_use_bluestein(vdev):
+    # 1000 = 2^3 5^3 and 7 * 2^9 have no single-pass case in the batch kernel
+    assert info(vdev, [1000])["levels"] == 1
+    assert info(vdev, [7 * 2**9])["levels"] == 4
+
+
+def test_knobs_change_the_chains(vdev):
+    assert info(vdev, [4096], single_max=1024)["levels"] == 2
+    assert info(vdev, [997], blue_max=1024)["levels"] == 4
+    assert info(vdev, [65536], factor_max=64)["levels"] == 3
+
+
+def test_tiles_cover_every_instance(vdev):
+    # 4096 x 3 copies: one instance per tile; 64 x 1: one tile of 64 slots
+    i = info(vdev, [4096, 64], [3, 1])
+    assert i["tiles"] == 3 + 1
+    assert i["items"] == 4
+
+
+def test_length_one_items_are_copies(vdev):
+    i = info(vdev, [1, 64])
+    assert i["launches"] == 2 and i["items"] == 2
+
+
+def test_grouped_mode(vdev):
+    i = info(vdev, [64, 65536, 997], mode="grouped")   # single-shape plans: 65536 = 256 * 256
+    assert i["launches"] == 1 + 2 + 1 and i["mode"] == 1
+
+
+def test_c4_workload_plans(vdev):
+    lengths = sample_lengths(512)
+    assert len(lengths) == 512 and min(lengths) >= 64 and max(lengths) <= 65536
+    i = info(vdev, lengths)
+    assert i["launches"] == 1 and i["items"] == 512
+    assert i["lds_bytes"] <= 64 * 1024
+
+
+def test_invalid_items(vdev):
+    with pytest.raises(ValueError):
+        BatchFFT(vdev, [BatchItem(0)])
+    with pytest.raises(ValueError):
+        BatchFFT(vdev, [BatchItem(64, count=2, in_dist=32, out_dist=64)])
+    with pytest.raises(ValueError):
+        BatchFFT(vdev, [BatchItem(64)], BatchOptions(mode="bogus"))
+
+
+def test_dry_batch_cannot_execute(vdev):
+    b = BatchFFT(vdev, [BatchItem(64)])
+    with pytest.raises(ValueError, match="VirtualDevice"):
+        b(None, None, None)
+
+
+def test_sample_lengths_families():
+    from reikna_amd.fft.batch import _is_prime
+
+    ls = sample_lengths(512, seed=5)
+    fam = {"pow2": 0, "3": 0, "5": 0, "prime": 0}
+    for n in ls:
+        if n & (n - 1) == 0:
+            fam["pow2"] += 1
+        elif n % 3 == 0 and (n // 3) & (n // 3 - 1) == 0:
+            fam["3"] += 1
+        elif n % 5 == 0 and (n // 5) & (n // 5 - 1) == 0:
+            fam["5"] += 1
+        else:
+            assert _is_prime(n)
+            fam["prime"] += 1
+    assert all(v > 64 for v in fam.values())
+    assert ls == sample_lengths(512, seed=5)

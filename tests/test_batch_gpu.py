@@ -1,0 +1,168 @@
+"""GPU parity of batched uneven transforms (BatchFFT) against the FP64 oracle.
+
+Every item of a batch is checked against numpy.fft in FP64 at the complex64
+bound (rel-L2 <= 2e-7 * log2 N).  Beyond parity: a transform computed alone
+and inside a mixed batch is bit-identical (its pass chain depends only on its
+length), HIP-graph replays match eager runs, repeated launches reuse the
+queue state, and the per-item path (grouped mode) agrees within tolerance.
+"""
+
+import numpy
+import pytest
+
+import oracle
+from reikna_amd import Array
+from reikna_amd.fft import BatchFFT, BatchItem, BatchOptions
+from reikna_amd.fft.batch import packed_items, sample_lengths
+
+pytestmark = pytest.mark.gpu
+
+SPECIAL = [64, 1000, 997, 4096, 8192, 65521, 49152, 40960, 2, 3, 7, 13, 1, 5003, 12289, 65536]
+
+
+def packed_data(lengths, counts, seed):
+    total = sum(n * c for n, c in zip(lengths, counts))
+    return oracle.test_array((total,), seed=seed)
+
+
+def check_items(items, host_in, host_out, inverse=False):
+    worst = 0.0
+    for it in items:
+        for j in range(it.count):
+            x = host_in[it.in_offset + j * it.in_dist: it.in_offset + j * it.in_dist + it.length]
+            y = host_out[it.out_offset + j * it.out_dist: it.out_offset + j * it.out_dist + it.length]
+            if inverse:
+                ref = oracle.ifftn(x, (0,))
+                if it.scale:
+                    ref = ref * it.length
+            else:
+                ref = oracle.fftn(x, (0,))
+                if it.scale:
+                    ref = ref / it.length
+            err = oracle.rel_l2(y, ref)
+            tol = oracle.tolerance((it.length,))
+            assert err <= tol, f"length {it.length} (item {it}, copy {j}): rel-L2 {err:.3g} > {tol:.3g}"
+            worst = max(worst, err / tol)
+    return worst
+
+
+def run_batch(queue, items, host_in, n_out, inverse=False, options=None, inplace=False):
+    batch = BatchFFT(queue.device, items, options)
+    inp = Array.from_host(queue, host_in)
+    out = inp if inplace else Array.from_host(queue, numpy.zeros(n_out, numpy.complex64))
+    batch(queue, out, inp, int(inverse))
+    res = out.get(queue)
+    batch.check()
+    return res, batch
+
+
+@pytest.mark.parametrize("inverse", [False, True])
+def test_mixed_batch_matches_oracle(queue, inverse):
+    rng = numpy.random.default_rng(21)
+    lengths = sample_lengths(40, seed=11) + SPECIAL
+    counts = [int(c) for c in rng.integers(1, 4, len(lengths))]
+    items = packed_items(lengths, counts)
+    data = packed_data(lengths, counts, seed=6)
+    res, batch = run_batch(queue, items, data, data.size, inverse=inverse)
+    assert batch.info["launches"] == 2   # the queue launch + the copy of the length-1 item
+    check_items(items, data, res, inverse=inverse)
+
+
+def test_scaled_items(queue):
+    lengths = [64, 997, 3 * 2**12, 65536, 65521]
+    items = packed_items(lengths, None, scale=True)
+    data = packed_data(lengths, [1] * len(lengths), seed=7)
+    fwd, _ = run_batch(queue, items, data, data.size)
+    check_items(items, data, fwd)
+    inv, _ = run_batch(queue, items, data, data.size, inverse=True)
+    check_items(items, data, inv, inverse=True)
+
+
+def test_strided_items(queue):
+    # distances larger than the length, separate in/out layouts, gaps
+    items = [
+        BatchItem(1024, 3, in_offset=5, out_offset=0, in_dist=1100, out_dist=1024),
+        BatchItem(997, 2, in_offset=4000, out_offset=3072, in_dist=1000, out_dist=2000),
+        BatchItem(8192, 2, in_offset=7000, out_offset=7072, in_dist=9000, out_dist=8192),
+    ]
+    n_in = 7000 + 9000 + 8192
+    n_out = 7072 + 8192 * 2
+    data = oracle.test_array((n_in,), seed=8)
+    res, _ = run_batch(queue, items, data, n_out)
+    check_items(items, data, res)
+
+
+def test_inplace(queue):
+    lengths = [256, 1000, 4099, 16384, 3 * 2**13]
+    counts = [2, 1, 2, 1, 1]
+    items = packed_items(lengths, counts)
+    data = packed_data(lengths, counts, seed=9)
+    res, _ = run_batch(queue, items, data, data.size, inplace=True)
+    check_items(items, data, res)
+
+
+def test_alone_and_in_mix_identical(queue):
+    lengths = sample_lengths(24, seed=12) + [997, 65521, 8192]
+    items = packed_items(lengths)
+    data = packed_data(lengths, [1] * len(lengths), seed=10)
+    mixed, _ = run_batch(queue, items, data, data.size)
+    for it in items[::3] + items[-3:]:
+        x = data[it.in_offset: it.in_offset + it.length]
+        alone, _ = run_batch(queue, [BatchItem(it.length)], x, it.length)
+        assert numpy.array_equal(alone, mixed[it.out_offset: it.out_offset + it.length]), f"length {it.length}"
+
+
+def test_grouped_mode_agrees(queue):
+    lengths = sample_lengths(16, seed=13) + [997, 12289]
+    items = packed_items(lengths)
+    data = packed_data(lengths, [1] * len(lengths), seed=11)
+    q, bq = run_batch(queue, items, data, data.size)
+    g, bg = run_batch(queue, items, data, data.size, options=BatchOptions(mode="grouped"))
+    assert bq.info["launches"] == 1 and bg.info["launches"] >= len(lengths)
+    check_items(items, data, g)
+    for it in items:
+        a = q[it.out_offset: it.out_offset + it.length]
+        b = g[it.out_offset: it.out_offset + it.length]
+        assert oracle.rel_l2(a, b.astype(numpy.complex128)) <= 2 * oracle.tolerance((it.length,))
+
+
+def test_repeated_launches_and_graph_replay(queue):
+    lengths = sample_lengths(32, seed=14) + [65521, 65536]
+    items = packed_items(lengths)
+    data = packed_data(lengths, [1] * len(lengths), seed=12)
+    batch = BatchFFT(queue.device, items)
+    inp = Array.from_host(queue, data)
+    out = Array.from_host(queue, numpy.zeros_like(data))
+    batch(queue, out, inp)
+    eager = out.get(queue)
+    check_items(items, data, eager)
+    # back-to-back launches without a host sync in between
+    for _ in range(5):
+        batch(queue, out, inp)
+    assert numpy.array_equal(out.get(queue), eager)
+    # one capture, several replays with eager work and syncs in between
+    with queue.capture() as cap:
+        batch(queue, out, inp)
+    other = Array.from_host(queue, numpy.zeros_like(data))
+    for _ in range(3):
+        out.set(queue, numpy.zeros_like(data))
+        cap.graph.launch(queue)
+        assert numpy.array_equal(out.get(queue), eager)
+        batch(queue, other, inp)
+        queue.synchronize()
+        assert numpy.array_equal(other.get(queue), eager)
+    batch.check()
+
+
+def test_autotune_small(queue):
+    from reikna_amd.fft.autotune import autotune_batch
+
+    lengths = sample_lengths(16, seed=15)
+    items = packed_items(lengths)
+    cands = [BatchOptions(), BatchOptions(single_max=1024, factor_max=64), BatchOptions(order=1)]
+    best, results = autotune_batch(queue.device, items, queue, candidates=cands, reps=3, use_cache=False)
+    assert len(results) == 3 and best in cands
+    assert all(ms > 0 for _, ms in results)
+    data = packed_data(lengths, [1] * len(lengths), seed=13)
+    res, _ = run_batch(queue, items, data, data.size, options=best)
+    check_items(items, data, res)
