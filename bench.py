@@ -40,6 +40,9 @@ WORKLOADS = {
     # batched uneven lengths: handled by bench_batch
     "c4": (None, None, "uniform", 6, "512 c64 C2C transforms, lengths log-uniform in [64, 65536] "
            "(2^k, 3*2^k, 5*2^k, primes; seed 5), fused 1/N, one launch"),
+    # request serving: handled by bench_serve
+    "c5": (None, None, "normal", 8, "Poisson stream of c64 FFT requests: 1-D 2^8..2^20 and 2-D 128^2..2048^2, "
+           "batch 1..64 (log-uniform, <= 2^25 elements), arrivals seed 7"),
 }
 
 # HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE
@@ -104,6 +107,10 @@ def main(argv=None) -> int:
     ap.add_argument("--workload", default="c2b", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--autotune", action="store_true", help="c4: pick batch planner knobs by measurement")
+    ap.add_argument("--requests", type=int, default=400, help="c5: requests per rate point")
+    ap.add_argument("--rates", default="1000,2000,5000,10000,20000,50000", help="c5: offered rates (requests/s)")
+    ap.add_argument("--small-rates", default="5000,20000,50000,100000,200000",
+                    help="c5: offered rates of the small-request stream")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(
@@ -122,6 +129,8 @@ def main(argv=None) -> int:
 
     if args.workload == "c4":
         return bench_batch(args)
+    if args.workload == "c5":
+        return bench_serve(args)
     shape, axes, dist, seed, desc = WORKLOADS[args.workload]
     dev = Device(0)
     queue = Queue(dev)
@@ -361,6 +370,126 @@ def bench_batch(args) -> int:
             f"min of {len(times)} runs",
         }
     print(json.dumps(out_json), flush=True)
+    return 0 if worst <= 1.0 else 1
+
+
+def bench_serve(args) -> int:
+    """A Poisson request stream at several rates, through the continuous-
+    batching server and through the plain per-request path (same admission,
+    one plan execution per request)."""
+    import numpy
+
+    import oracle
+    from reikna_amd import Queue
+    from reikna_amd.fft.serve import FFTServer, make_specs, run_stream, sample_shapes
+    from reikna_amd.runtime import Device
+
+    _, _, dist, seed, desc = WORKLOADS["c5"]
+    dev = Device(0)
+    queue = Queue(dev)
+    n_req = max(50, args.requests)
+    shapes = sample_shapes(n_req, seed=7)
+    specs = make_specs(queue, shapes, seed=seed)
+    rates = [float(r) for r in args.rates.split(",")]
+    print(f"[bench] {desc}: {n_req} requests per rate, rates {rates}", file=sys.stderr, flush=True)
+    budget = 16 * 2**30
+    plain_min = 2**22
+    servers = {
+        "batched": FFTServer(dev, budget, slots=4, mode="batched", plain_min_elements=plain_min),
+        "plain": FFTServer(dev, budget, slots=4, mode="plain"),
+    }
+    # warm up: every shape's template / plan built, kernels loaded
+    for srv in servers.values():
+        run_stream(srv, specs, rate=1e6, seed=1)
+    # correctness: a sample of the batched server's outputs against the oracle
+    host_inputs = {}
+    worst = 0.0
+    sample_ids = set(range(0, n_req, max(1, n_req // 24)))
+
+    def check(c, spec):
+        nonlocal worst
+        if c.id not in sample_ids:
+            return
+        key = id(spec.input)
+        if key not in host_inputs:
+            host_inputs[key] = spec.input.get(queue)
+        x = host_inputs[key][: spec.elements].reshape(spec.count, *spec.shape)
+        ref = oracle.fftn(x, tuple(range(1, 1 + len(spec.shape))))
+        out = servers["batched"].output(c, spec).get(queue)
+        worst = max(worst, oracle.rel_l2(out, ref) / oracle.tolerance(spec.shape))
+
+    run_stream(servers["batched"], specs, rate=1e6, seed=2, on_complete=check)
+    sweep = {}
+    for mode, srv in servers.items():
+        rows = []
+        for r in rates:
+            res = run_stream(srv, specs, rate=r, seed=7)
+            rows.append({"rate": r, "throughput_rps": round(res.throughput_rps, 1), "gflops": round(res.gflops, 1),
+                         "p50_ms": round(res.p50_ms, 3), "p99_ms": round(res.p99_ms, 3),
+                         "launches": res.launches})
+            print(f"[bench] {mode:7s} rate {r:8.0f}/s -> {res.throughput_rps:8.1f} req/s, {res.gflops:8.1f} GFLOP/s, "
+                  f"p50 {res.p50_ms:.3f} ms, p99 {res.p99_ms:.3f} ms, {res.launches} launches",
+                  file=sys.stderr, flush=True)
+        sweep[mode] = rows
+    best = {m: max(rows, key=lambda x: x["throughput_rps"]) for m, rows in sweep.items()}
+    total_flops = sum(sp.flops for sp in specs)
+    # the same servers on a stream of small requests (1-D 2^6..2^12, 2-D
+    # 32^2..128^2, batch 1..16), where per-request launches dominate the
+    # plain path
+    small_specs = make_specs(queue, sample_shapes(n_req, seed=9, one_d=(6, 12), two_d=(5, 7), max_count=16), seed=10)
+    for srv in servers.values():
+        run_stream(srv, small_specs, rate=1e6, seed=1)
+    small_rates = [float(r) for r in args.small_rates.split(",")]
+    small = {}
+    for mode, srv in servers.items():
+        rows = []
+        for r in small_rates:
+            res = run_stream(srv, small_specs, rate=r, seed=11)
+            rows.append({"rate": r, "throughput_rps": round(res.throughput_rps, 1), "gflops": round(res.gflops, 1),
+                         "p50_ms": round(res.p50_ms, 3), "p99_ms": round(res.p99_ms, 3),
+                         "launches": res.launches})
+            print(f"[bench] small {mode:7s} rate {r:8.0f}/s -> {res.throughput_rps:8.1f} req/s, "
+                  f"p50 {res.p50_ms:.3f} ms, p99 {res.p99_ms:.3f} ms, {res.launches} launches",
+                  file=sys.stderr, flush=True)
+        small[mode] = rows
+    small_best = {m: max(rows, key=lambda x: x["throughput_rps"]) for m, rows in small.items()}
+    out = {
+        "metric": METRIC,
+        "value": best["batched"]["throughput_rps"],
+        "unit": "requests/s",
+        "n_gpus": 1,
+        "steps": len(rates),
+        "warmup": 1,
+        "ms_per_step": round(1e3 / best["batched"]["throughput_rps"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {
+            "workload": desc,
+            "requests_per_rate": n_req,
+            "mean_elements_per_request": round(sum(sp.elements for sp in specs) / n_req),
+            "mean_gflop_per_request": round(total_flops / n_req / 1e9, 4),
+            "budget_bytes": budget,
+            "slots": 4,
+            "plain_min_elements": plain_min,
+        },
+        "sweep": sweep,
+        "best": best,
+        "speedup_vs_plain": round(best["batched"]["throughput_rps"] / best["plain"]["throughput_rps"], 2),
+        "small_mix": {
+            "workload": "1-D 2^6..2^12 and 2-D 32^2..128^2, batch 1..16 (log-uniform), arrivals seed 11",
+            "sweep": small,
+            "best": small_best,
+            "speedup_vs_plain": round(small_best["batched"]["throughput_rps"] / small_best["plain"]["throughput_rps"],
+                                      2),
+        },
+        "worst_rel_l2_over_bound": round(worst, 4),
+    }
+    for srv in servers.values():
+        srv.close()
+    print(json.dumps(out), flush=True)
     return 0 if worst <= 1.0 else 1
 
 

@@ -253,8 +253,10 @@ typedef struct {
     uint32_t flags;       /* RK_BATCH_DRY_RUN */
     int32_t single_max;   /* longest single-pass length (0: 4096, at most 4096) */
     int32_t factor_max;   /* longest pass of a multi-pass chain (0: 256, at most 256) */
-    int32_t blue_max;     /* longest single-kernel Bluestein length (0: 2048, at most 2048) */
-    int32_t order;        /* queue order inside a level: 0 largest tiles first, 1 item order */
+    int32_t blue_max;     /* reserved, 0 (Bluestein runs as two chains in the batch kernel) */
+    int32_t order;        /* queue order: 0 critical path (longest chains first, all chains
+                             ending together), 1 level by level in item order,
+                             2 level by level, largest tiles first */
 } rk_batch_options;
 
 typedef struct {
@@ -266,6 +268,8 @@ typedef struct {
     int32_t launches;     /* kernel launches per execute */
     int32_t mode;
     int32_t lds_bytes;    /* LDS per work-group of the queue kernel */
+    int32_t flavour;      /* 0: light kernel, 1: kernel with single-kernel Bluestein cases */
+    int32_t _reserved;
 } rk_batch_info;
 
 int rk_batch_plan_create(const rk_batch_item* items, int n_items,
@@ -286,6 +290,58 @@ int rk_batch_plan_info(const rk_batch_plan* plan, rk_batch_info* info);
    after its poll limit (the results of that execute are not valid); clears
    the flag. */
 int rk_batch_plan_check(rk_batch_plan* plan, int* hit);
+
+/* ------------------------------------------------------------------ */
+/* Request server — continuous batching of FFT requests (no reference   */
+/* counterpart: reikna has no serving layer).                           */
+/* ------------------------------------------------------------------ */
+typedef struct rk_server rk_server;
+
+#define RK_SERVER_BATCHED 0u   /* admitted requests run together in one batch-kernel launch */
+#define RK_SERVER_PLAIN 1u     /* each request runs by itself (its single-shape plan) */
+
+typedef struct {
+    size_t budget_bytes;   /* HBM for request outputs and workspaces */
+    int32_t slots;         /* launches in flight at once (one stream each), 1..16 */
+    int32_t max_requests;  /* requests per launch (0: 1024) */
+    uint32_t mode;         /* RK_SERVER_* */
+    int64_t plain_min_elements;  /* RK_SERVER_BATCHED: requests of at least this many
+                                    elements run by themselves with their single-shape
+                                    plan (0: never) */
+} rk_server_config;
+
+typedef struct {
+    uint64_t id;           /* caller's tag, unique among unreleased requests */
+    int32_t ndim;          /* 1 or 2: the rank of each transform */
+    int32_t inverse;
+    int64_t shape[2];      /* (n) or (n0, n1), row-major, last axis contiguous */
+    int64_t count;         /* transforms in the request, back to back */
+    const void* in;        /* device input, count * prod(shape) complex64 */
+    uint64_t t_arrival_ns; /* rk_server_now_ns() clock; 0: time of submit */
+} rk_request;
+
+typedef struct {
+    uint64_t id;
+    void* out;             /* device output, valid until rk_server_release(id) */
+    uint64_t t_arrival_ns, t_admitted_ns, t_done_ns;
+} rk_completion;
+
+typedef struct {
+    int64_t pending, in_flight_requests, in_flight_launches, completed, launches, templates;
+    size_t budget_used, budget_bytes;
+} rk_server_stats_t;
+
+int rk_server_create(const rk_server_config* config, rk_server** server);
+/* Queue requests (they wait for memory and a slot); RK_ERR_BUDGET for a
+   request that can never fit in the budget. */
+int rk_server_submit(rk_server* server, const rk_request* requests, int n);
+/* Reap finished launches, admit and launch queued requests, and return up to
+   max_done completions (never blocks). */
+int rk_server_progress(rk_server* server, rk_completion* done, int max_done, int* n_done);
+int rk_server_release(rk_server* server, uint64_t id);
+int rk_server_stats(const rk_server* server, rk_server_stats_t* stats);
+uint64_t rk_server_now_ns(void);
+int rk_server_destroy(rk_server* server);
 
 /* ------------------------------------------------------------------ */
 /* HIP graphs: capture a stream's work once, replay it with one call.   */

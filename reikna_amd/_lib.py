@@ -33,6 +33,54 @@ RK_FFT_NORM_FORWARD = 0x8
 RK_FFT_OUT_LAST_ONLY = 0x10
 RK_FFT_DRY_RUN = 0x20
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("budget_bytes", ctypes.c_size_t),
+        ("slots", ctypes.c_int32),
+        ("max_requests", ctypes.c_int32),
+        ("mode", ctypes.c_uint32),
+        ("plain_min_elements", ctypes.c_int64),
+    ]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_uint64),
+        ("ndim", ctypes.c_int32),
+        ("inverse", ctypes.c_int32),
+        ("shape", ctypes.c_int64 * 2),
+        ("count", ctypes.c_int64),
+        ("input", ctypes.c_void_p),
+        ("t_arrival_ns", ctypes.c_uint64),
+    ]
+
+
+class Completion(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_uint64),
+        ("out", ctypes.c_void_p),
+        ("t_arrival_ns", ctypes.c_uint64),
+        ("t_admitted_ns", ctypes.c_uint64),
+        ("t_done_ns", ctypes.c_uint64),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("pending", ctypes.c_int64),
+        ("in_flight_requests", ctypes.c_int64),
+        ("in_flight_launches", ctypes.c_int64),
+        ("completed", ctypes.c_int64),
+        ("launches", ctypes.c_int64),
+        ("templates", ctypes.c_int64),
+        ("budget_used", ctypes.c_size_t),
+        ("budget_bytes", ctypes.c_size_t),
+    ]
+
+
+RK_SERVER_BATCHED = 0
+RK_SERVER_PLAIN = 1
+
 RK_BATCH_SCALE = 0x1
 
 
@@ -136,6 +184,8 @@ class BatchInfo(ctypes.Structure):
         ("launches", ctypes.c_int32),
         ("mode", ctypes.c_int32),
         ("lds_bytes", ctypes.c_int32),
+        ("flavour", ctypes.c_int32),
+        ("_reserved", ctypes.c_int32),
     ]
 
 
@@ -200,6 +250,13 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "rk_batch_plan_create_ex": (_I, [ctypes.POINTER(BatchItem), _I, ctypes.POINTER(BatchOptions), _PP]),
     "rk_batch_plan_info": (_I, [_P, ctypes.POINTER(BatchInfo)]),
     "rk_batch_plan_check": (_I, [_P, ctypes.POINTER(_I)]),
+    "rk_server_create": (_I, [ctypes.POINTER(ServerConfig), _PP]),
+    "rk_server_submit": (_I, [_P, ctypes.POINTER(Request), _I]),
+    "rk_server_progress": (_I, [_P, ctypes.POINTER(Completion), _I, ctypes.POINTER(_I)]),
+    "rk_server_release": (_I, [_P, ctypes.c_uint64]),
+    "rk_server_stats": (_I, [_P, ctypes.POINTER(ServerStats)]),
+    "rk_server_now_ns": (ctypes.c_uint64, []),
+    "rk_server_destroy": (_I, [_P]),
     "rk_graph_begin": (_I, [_P]),
     "rk_graph_end": (_I, [_P, _PP]),
     "rk_graph_launch": (_I, [_P, _P]),

@@ -147,20 +147,31 @@ def batch_tile_c(length: int) -> int:
     return max(1, BATCH_THREADS // threads_per_instance(length))
 
 
+VAR_CHIRP_KMUL = 2 | (1 << 2)        # single-pass first Bluestein chain
+VAR_CONJ_CHIRP = 2 << 2               # single-pass second Bluestein chain
+
+
 BATCH_BLUE_MAX = 2048
 
 
-def batch_entries() -> list[tuple[int, int, int]]:
+def batch_entries(blue: bool = True) -> list[tuple[int, int, int]]:
     """Tile cases of the batch kernel.  Bluestein runs over power-of-two
-    lengths only (single kernel up to BATCH_BLUE_MAX, two chains above), and
-    multi-pass chains use pass lengths 16..BATCH_FACTOR_MAX."""
+    lengths: as one single-kernel pass up to BATCH_BLUE_MAX in the blue
+    flavour, else as two chains (one pass each up to BATCH_SINGLE_MAX, passes
+    of 16..BATCH_FACTOR_MAX above); multi-pass chains use pass lengths
+    16..BATCH_FACTOR_MAX.  The single-kernel Bluestein cases need ~180-250
+    VGPRs, and a kernel's register count (its largest case's) sets the
+    occupancy of every case, so the light flavour leaves them out."""
     ents = set()
     for length in batch_lengths():
         c = batch_tile_c(length)
         pow2 = length & (length - 1) == 0
         ents.add((length, c, VAR_PLAIN))
-        if pow2 and 8 <= length <= BATCH_BLUE_MAX:
-            ents.add((length, c, VAR_BLUE))
+        if pow2 and length >= 16:
+            ents.add((length, c, VAR_CHIRP_KMUL))
+            ents.add((length, c, VAR_CONJ_CHIRP))
+            if blue and length <= BATCH_BLUE_MAX:
+                ents.add((length, c, VAR_BLUE))
         if 16 <= length <= BATCH_FACTOR_MAX:
             ents.add((length, c, VAR_TWIDDLE))
             if pow2:
@@ -169,18 +180,21 @@ def batch_entries() -> list[tuple[int, int, int]]:
     return sorted(ents)
 
 
-def generate_batch_cases(out_dir: Path) -> Path:
-    lines = [
-        "// generated by reikna_amd/csrc/build.py: tile cases of the persistent batch kernel",
-        "// X(case id, L, C, variant)",
-        "#define RK_BATCH_CASES(X) \\",
-    ]
-    for k, (length, c, v) in enumerate(batch_entries()):
-        lines.append(f"    X({k}, {length}, {c}, {v}) \\")
-    lines.append("")
-    path = out_dir / "batch_cases.inc"
-    write_if_changed(path, "\n".join(lines) + "\n")
-    return path
+def generate_batch_cases(out_dir: Path) -> list[Path]:
+    paths = []
+    for flavour in ("light", "blue"):
+        lines = [
+            f"// generated by reikna_amd/csrc/build.py: tile cases of the batch kernel ({flavour})",
+            "// X(case id, L, C, variant)",
+            "#define RK_BATCH_CASES(X) \\",
+        ]
+        for k, (length, c, v) in enumerate(batch_entries(blue=flavour == "blue")):
+            lines.append(f"    X({k}, {length}, {c}, {v}) \\")
+        lines.append("")
+        path = out_dir / f"batch_cases_{flavour}.inc"
+        write_if_changed(path, "\n".join(lines) + "\n")
+        paths.append(path)
+    return paths
 
 
 def variant_cost(v: int) -> float:
@@ -311,12 +325,15 @@ def build(jobs: int = 8, incremental: bool = False, arch: str = "gfx950", verbos
     gen = generate_catalog(BUILD)
     generate_embedded(BUILD)
     generate_batch_cases(BUILD)
+    batch_kernels = [HERE / "rk_batch_blue.hip", HERE / "rk_batch_light.hip"]
     hand = [HERE / "rk_runtime.cpp", HERE / "rk_fft_plan.cpp", HERE / "rk_jit.cpp", HERE / "rk_copy.hip",
-            HERE / "rk_batch.cpp", HERE / "rk_batch_kernel.hip"]
+            HERE / "rk_batch.cpp", HERE / "rk_serve.cpp"]
     deps = [HERE / "fft_args.h", HERE / "fft_device.hpp", HERE / "fft_batch.h", HERE / "rk_internal.hpp",
-            REPO / "include" / "reikna_amd.h", BUILD / "rk_embedded.inc", BUILD / "batch_cases.inc"]
-    # the batch kernel is the longest compile: start it first
-    sources = [HERE / "rk_batch_kernel.hip"] + [p for p in hand if p.name != "rk_batch_kernel.hip"] + gen
+            HERE / "rk_batch_internal.hpp", HERE / "rk_batch_kernel.hpp",
+            REPO / "include" / "reikna_amd.h", BUILD / "rk_embedded.inc",
+            BUILD / "batch_cases_light.inc", BUILD / "batch_cases_blue.inc"]
+    # the batch kernels are the longest compiles: start them first
+    sources = batch_kernels + hand + gen
     t0 = time.time()
     objs = []
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:

@@ -3,12 +3,14 @@
 // A batch plan turns every transform of a batch into a chain of passes
 // (rk_pass_args, see fft_args.h) and every pass into tiles.  All tiles of all
 // passes form one queue, ordered level by level (every first pass, then every
-// second pass, ...).  Each work-group of the launch takes the next queue
-// position from an atomic counter as it starts; a tile of a pass that reads
-// the output of an earlier pass waits until all tiles of that pass are
-// counted done.  Every pass a tile can wait for sits earlier in the queue, so
-// its tiles were taken by work-groups that are already running, and every
-// wait ends (a poll limit turns a wait that does not end into an error flag).
+// second pass, ...).  Work-group b of the launch runs queue position b; a
+// tile of a pass that reads the output of an earlier pass waits until all
+// tiles of that pass are counted done.  Every pass a tile can wait for sits
+// earlier in the queue, and work-groups are dispatched in index order, so the
+// awaited tiles belong to work-groups that are running or finished and every
+// wait ends (a poll limit turns a wait that does not end into an error flag
+// instead of a hang).  The last tile of a pass to pass its wait clears the
+// producer's counters, so they are zero again for the next launch.
 //
 // Shared by the host planner and device code, so it uses only builtin types.
 #ifndef RK_FFT_BATCH_H
@@ -21,6 +23,7 @@
 #define RK_BATCH_BUF_IN 0u
 #define RK_BATCH_BUF_OUT 1u
 #define RK_BATCH_BUF_WS 2u
+#define RK_BATCH_BUF_ABS 3u   // the pointer already in the pass arguments
 
 struct rk_batch_pass {
     rk_pass_args a;          // in/out are taken from the launch (src/dst below)
@@ -34,15 +37,15 @@ struct rk_batch_pass {
     rk_u32 item_last;        // last pass of its transform (inverse: conjugate output)
     float fwd_scale;         // output scale of the forward transform
     float inv_scale;         // output scale of the inverse transform
-    rk_u32 _pad;
+    rk_u32 inverse;          // 1: this pass belongs to an inverse transform
 };
 
 struct rk_batch_work {
     const rk_batch_pass* passes;
     const rk_u32* tile_pass;  // pass of each queue position
     rk_u32* done;             // finished tiles per pass (zero between launches)
-    rk_u32* state;            // [0] next queue position, [1] work-groups finished,
-                              // [2] set when a wait gave up after spin_limit polls
+    rk_u32* seen;             // tiles of the consuming pass past their wait, per pass
+    rk_u32* state;            // [2]: set when a wait gave up after spin_limit polls
     const float2* in;
     float2* out;
     float2* ws;

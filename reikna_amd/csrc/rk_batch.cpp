@@ -1,5 +1,6 @@
 // rk_batch.cpp — batched transforms of uneven lengths behind
-// rk_batch_plan_create / rk_batch_execute.
+// rk_batch_plan_create / rk_batch_execute, and the pass chains of the batch
+// kernel (rk_batch_internal.hpp) that the request server reuses.
 //
 // No reference counterpart: reikna compiles and launches one computation per
 // shape (reikna/fft/fft.py:723-756).  Two execution modes:
@@ -17,8 +18,7 @@
 #include <memory>
 #include <numeric>
 
-#include "rk_internal.hpp"
-#include "fft_batch.h"
+#include "rk_batch_internal.hpp"
 
 using rk::fail;
 
@@ -26,11 +26,11 @@ namespace {
 
 constexpr int64_t kBatchSingleMax = 4096;
 constexpr int64_t kBatchFactorMax = 256;
-constexpr int64_t kBatchBlueMax = 2048;
+constexpr int64_t kBatchStridedMax = 256;   // column axes: >= 16 adjacent instances per tile
 constexpr uint32_t kSpinLimit = 1u << 24;   // ~2 s of 256-cycle sleeps
 
-const rk::BatchCase* find_case(int L, int variant) {
-    for (const auto& c : rk::batch_cases())
+const rk::BatchCase* find_case(int L, int variant, int flavour = rk::BATCH_BLUE) {
+    for (const auto& c : rk::batch_cases(flavour))
         if (c.L == L && c.variant == variant) return &c;
     return nullptr;
 }
@@ -42,26 +42,191 @@ int batch_tile_width(int L, bool) {
     return c ? c->C : 1;
 }
 
-struct ItemPass {
-    rk::PassPlan pp;
-    rk::AxisPass ap;
-    int item = 0, level = 0, kcase = -1;
-    bool first = false, last = false;
-    float fwd_scale = 1.0f, inv_scale = 1.0f;
-};
-
 }  // namespace
+
+namespace rk {
+
+ChainLimits batch_limits(const rk_batch_options* o) {
+    ChainLimits lim;
+    const int32_t sm = o ? o->single_max : 0, fm = o ? o->factor_max : 0;
+    lim.single_max = sm > 0 ? std::min<int64_t>(sm, kBatchSingleMax) : kBatchSingleMax;
+    lim.factor_max = fm > 0 ? std::min<int64_t>(fm, kBatchFactorMax) : kBatchFactorMax;
+    // single-kernel Bluestein up to the longest power of two with a case
+    static const int64_t blue_max = [] {
+        int64_t m = 0;
+        for (const auto& c : batch_cases(BATCH_BLUE))
+            if (c.variant == VAR_BLUE && (c.L & (c.L - 1)) == 0) m = std::max<int64_t>(m, c.L);
+        return m;
+    }();
+    lim.blue_single_max = blue_max;
+    lim.pow2_bluestein = true;
+    lim.length_ok = batch_length_ok;
+    lim.strided_single_max = kBatchStridedMax;
+    return lim;
+}
+
+int build_chain(const ItemShape& it, const ChainLimits& lim, TableBuilder& tb, Chain& chain) {
+    rk_fft_desc d;
+    memset(&d, 0, sizeof(d));
+    d.dtype = RK_C64;
+    d.ndim = 1 + it.nd;
+    d.shape[0] = it.count;
+    d.in_strides[0] = it.in_dist;
+    d.out_strides[0] = it.out_dist;
+    int64_t inner = 1;
+    for (int k = it.nd; k >= 1; --k) {
+        d.shape[k] = it.n[k - 1];
+        d.in_strides[k] = inner;
+        d.out_strides[k] = inner;
+        inner *= it.n[k - 1];
+    }
+    d.in_offset = it.in_offset;
+    d.out_offset = it.out_offset;
+    Layout lin, lout;
+    lin.ndim = lout.ndim = d.ndim;
+    for (int k = 0; k < d.ndim; ++k) {
+        lin.shape[k] = lout.shape[k] = d.shape[k];
+        lin.strides[k] = d.in_strides[k];
+        lout.strides[k] = d.out_strides[k];
+    }
+    lin.offset = d.in_offset;
+    lout.offset = d.out_offset;
+    std::vector<int> work;   // innermost first
+    for (int k = d.ndim - 1; k >= 1; --k)
+        if (d.shape[k] > 1) work.push_back(k);
+    if (work.empty()) return fail(RK_ERR_INVALID, "batch: a transform of one element is a copy");
+    // a smooth length whose passes have no tile case goes through Bluestein
+    int rc = RK_OK;
+    for (uint32_t force : {0u, (uint32_t)RK_FFT_FORCE_BLUESTEIN}) {
+        d.flags = force;
+        std::vector<PassPlan> passes;
+        std::vector<AxisPass> aps;
+        size_t ws = 0;
+        bool two = false;
+        PassHook hook = [&](PassPlan& pp, const AxisPass& ap, int64_t) -> int {
+            const BatchCase* bc = find_case(ap.L, pp.variant);
+            if (!bc || bc->C != pp.C || !pp.loops.empty())
+                return fail(RK_ERR_NO_KERNEL, "batch: no tile case for L=%d variant %d", ap.L, pp.variant);
+            aps.push_back(ap);
+            return RK_OK;
+        };
+        rc = route_passes(d, work, lin, lout, lim, batch_tile_width, hook, passes, ws, two);
+        if (rc != RK_OK) continue;
+        chain.passes.clear();
+        chain.ws_elems = ws;
+        chain.two_ws = two;
+        chain.blue = false;
+        for (const PassPlan& pp : passes)
+            if (pp.src == BUF_WS1 || pp.dst == BUF_WS1) chain.two_ws = true;
+        const double norm = 1.0 / (double)it.elements();
+        for (size_t i = 0; i < passes.size(); ++i) {
+            ChainPass cp;
+            cp.pp = passes[i];
+            cp.ap = aps[i];
+            cp.first = i == 0;
+            if (cp.pp.variant == VAR_BLUE) chain.blue = true;
+            cp.last = i + 1 == passes.size();
+            if (cp.last) {
+                cp.fwd_scale = it.scale ? (float)norm : 1.0f;
+                cp.inv_scale = it.scale ? 1.0f : (float)norm;
+            }
+            // tables from the axis length before Bluestein padding
+            rc = build_tables(tb, cp.pp, cp.ap, d.shape[cp.pp.axis]);
+            if (rc != RK_OK) return rc;
+            chain.passes.push_back(cp);
+        }
+        return RK_OK;
+    }
+    return rc;
+}
+
+uint64_t assemble_queue(const std::vector<QueuedPass>& in, int order_mode, int flavour,
+                        std::vector<rk_batch_pass>& out, std::vector<rk_u32>& tile_pass, int& levels) {
+    // chain length of every pass's item (the passes of an item are consecutive)
+    std::vector<int> rank(in.size());
+    int longest = 0;
+    for (size_t i = 0; i < in.size();) {
+        size_t j = i;
+        while (j < in.size() && in[j].item == in[i].item) ++j;
+        longest = std::max<int>(longest, (int)(j - i));
+        i = j;
+    }
+    for (size_t i = 0; i < in.size();) {
+        size_t j = i;
+        while (j < in.size() && in[j].item == in[i].item) ++j;
+        const int shift = order_mode == 0 ? longest - (int)(j - i) : 0;
+        for (size_t k = i; k < j; ++k) rank[k] = in[k].level + shift;
+        i = j;
+    }
+    std::vector<size_t> order(in.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        const QueuedPass &x = in[a], &y = in[b];
+        if (rank[a] != rank[b]) return rank[a] < rank[b];
+        if (order_mode == 1) return x.item < y.item;
+        const int64_t tx = (int64_t)x.cp->pp.C * x.cp->pp.L, ty = (int64_t)y.cp->pp.C * y.cp->pp.L;
+        if (tx != ty) return tx > ty;
+        return x.item < y.item;
+    });
+    std::vector<int> pos(in.size());
+    for (size_t q = 0; q < order.size(); ++q) pos[order[q]] = (int)q;
+    uint64_t total = 0;
+    levels = 0;
+    out.resize(in.size());
+    for (size_t q = 0; q < order.size(); ++q) {
+        const size_t i = order[q];
+        const QueuedPass& qp = in[i];
+        const ChainPass& cp = *qp.cp;
+        rk_batch_pass& bp = out[q];
+        memset(&bp, 0, sizeof(bp));
+        bp.a = qp.a;
+        bp.first = total;
+        const BatchCase* bc = find_case(cp.pp.L, cp.pp.variant, flavour);
+        bp.kcase = bc ? (rk_u32)bc->id : 0u;   // build_chain checked the case exists in the blue flavour
+        bp.src = qp.src;
+        bp.dst = qp.dst;
+        bp.wait = cp.first ? -1 : pos[i - 1];   // the item's previous pass
+        bp.wait_tiles = cp.first ? 0u : (rk_u32)in[i - 1].a.n_tiles;
+        bp.signal = cp.last ? 0u : 1u;
+        bp.item_first = cp.first ? 1u : 0u;
+        bp.item_last = cp.last ? 1u : 0u;
+        bp.fwd_scale = cp.fwd_scale;
+        bp.inv_scale = cp.inv_scale;
+        bp.inverse = qp.inverse;
+        total += qp.a.n_tiles;
+        levels = std::max(levels, qp.level + 1);
+    }
+    tile_pass.resize(total);
+    for (size_t q = 0; q < out.size(); ++q)
+        for (rk_u64 t = 0; t < out[q].a.n_tiles; ++t) tile_pass[out[q].first + t] = (rk_u32)q;
+    return total;
+}
+
+void resolve_tables(rk_pass_args& a, const PassPlan& pp, const char* tables) {
+    auto tab = [&](int64_t off) -> const void* { return off < 0 || !tables ? nullptr : tables + off; };
+    a.twl = (const float2*)tab(pp.twl_off);
+    a.tw_hi = (const double2*)tab(pp.twhi_off);
+    a.tw_lo = (const double2*)tab(pp.twlo_off);
+    a.chirp = (const float2*)tab(pp.chirp_off);
+    a.bspec = (const float2*)tab(pp.bspec_off);
+    a.tile_base = 0;
+    a.xcd_group = 0;
+}
+
+}  // namespace rk
 
 struct rk_batch_plan {
     rk_batch_options opt;
     // grouped mode
     std::vector<rk_fft_plan*> plans;
     // queue mode
+    std::vector<rk::Chain> chains;
     std::vector<rk_batch_pass> passes;   // queue order
     std::vector<rk_u32> tile_pass;
     std::vector<rk::CopyArgs> copies;    // length-1 items
-    void* dev = nullptr;                 // tables | passes | tile_pass | done | state
+    void* dev = nullptr;                 // tables | passes | tile_pass | done | seen | state
     rk_batch_work work;
+    int flavour = rk::BATCH_LIGHT;
     int64_t n_items = 0;
     int levels = 0;
     size_t ws_bytes = 0;
@@ -113,20 +278,15 @@ static int create_grouped(const rk_batch_item* items, int n_items, rk_batch_plan
 }
 
 static int create_queue(const rk_batch_item* items, int n_items, rk_batch_plan* plan) {
-    const rk_batch_options& o = plan->opt;
-    rk::ChainLimits lim;
-    lim.single_max = o.single_max > 0 ? std::min<int64_t>(o.single_max, kBatchSingleMax) : kBatchSingleMax;
-    lim.factor_max = o.factor_max > 0 ? std::min<int64_t>(o.factor_max, kBatchFactorMax) : kBatchFactorMax;
-    lim.blue_single_max = o.blue_max > 0 ? std::min<int64_t>(o.blue_max, kBatchBlueMax) : kBatchBlueMax;
-    lim.pow2_bluestein = true;
-    lim.length_ok = batch_length_ok;
-
+    const rk::ChainLimits lim = rk::batch_limits(&plan->opt);
     rk::TableBuilder tb;
-    std::vector<ItemPass> all;
+    std::vector<rk::QueuedPass> queued;
     size_t ws_elems = 0;
+    plan->chains.reserve(n_items);   // QueuedPass keeps pointers into the chains
     for (int k = 0; k < n_items; ++k) {
         const rk_batch_item& it = items[k];
         if (it.count == 0) continue;
+        plan->n_items += it.count;
         if (it.length == 1) {   // identity: a strided copy launched beside the queue
             rk::CopyArgs c;
             memset(&c, 0, sizeof(c));
@@ -140,165 +300,72 @@ static int create_queue(const rk_batch_item* items, int n_items, rk_batch_plan* 
             plan->copies.push_back(c);
             continue;
         }
-        // the item's chain, over the tile cases the batch kernel has; a smooth
-        // length whose passes have no case goes through Bluestein instead
-        std::vector<rk::AxisPass> aps;
-        std::vector<ItemPass> chain;
-        for (uint32_t force : {0u, (uint32_t)RK_FFT_FORCE_BLUESTEIN}) {
-            chain.clear();
-            int rc = rk::axis_passes(it.length, force, lim, aps);
-            if (rc != RK_OK) {
-                if (force) return rc;
-                continue;
-            }
-            bool ok = true;
-            const int64_t one = it.count;
-            size_t item_ws = 0;
-            for (size_t pi = 0; pi < aps.size() && ok; ++pi) {
-                const rk::AxisPass& ap = aps[pi];
-                ItemPass ip;
-                ip.ap = ap;
-                ip.item = k;
-                ip.level = (int)pi;
-                ip.first = pi == 0;
-                ip.last = pi + 1 == aps.size();
-                // src / dst: in -> ws0 -> ws1 -> ws0 ... -> out
-                const int64_t shape_in[2] = {one, ap.in_len}, shape_out[2] = {one, ap.out_len};
-                rk::Layout ls, ld;
-                if (ip.first) {
-                    ls.ndim = 2;
-                    ls.shape[0] = one;
-                    ls.shape[1] = ap.in_len;
-                    ls.strides[0] = it.in_dist;
-                    ls.strides[1] = 1;
-                    ls.offset = it.in_offset;
-                    ip.pp.src = rk::BUF_IN;
-                } else {
-                    ls = rk::contiguous_layout(2, shape_in);
-                    ls.offset = (int64_t)ws_elems + (int64_t)(((pi - 1) % 2) * (size_t)(one * ap.Nax));
-                    ip.pp.src = rk::BUF_WS0;
-                }
-                if (ip.last) {
-                    ld.ndim = 2;
-                    ld.shape[0] = one;
-                    ld.shape[1] = ap.out_len;
-                    ld.strides[0] = it.out_dist;
-                    ld.strides[1] = 1;
-                    ld.offset = it.out_offset;
-                    ip.pp.dst = rk::BUF_OUT;
-                } else {
-                    ld = rk::contiguous_layout(2, shape_out);
-                    ld.offset = (int64_t)ws_elems + (int64_t)((pi % 2) * (size_t)(one * ap.Nax));
-                    ip.pp.dst = rk::BUF_WS0;
-                    item_ws = std::max(item_ws, (size_t)((pi % 2) + 1) * (size_t)(one * ap.Nax));
-                }
-                int rc2 = rk::describe_pass(ap, ls, ld, 1, batch_tile_width, ip.pp);
-                if (rc2 != RK_OK) return rc2;
-                const rk::BatchCase* bc = find_case(ap.L, ip.pp.variant);
-                if (!bc || bc->C != ip.pp.C || !ip.pp.loops.empty()) {
-                    ok = false;
-                    break;
-                }
-                ip.kcase = bc->id;
-                chain.push_back(ip);
-            }
-            if (!ok) {
-                if (force) return fail(RK_ERR_NO_KERNEL, "batch item %d (length %lld): no tile case for its passes", k,
-                                       (long long)it.length);
-                continue;
-            }
-            ws_elems += (item_ws + 31) & ~size_t(31);   // 256-byte aligned regions
-            break;
+        rk::ItemShape sh;
+        sh.nd = 1;
+        sh.n[0] = it.length;
+        sh.count = it.count;
+        sh.in_offset = it.in_offset;
+        sh.out_offset = it.out_offset;
+        sh.in_dist = it.count > 1 ? it.in_dist : it.length;
+        sh.out_dist = it.count > 1 ? it.out_dist : it.length;
+        sh.scale = (it.flags & RK_BATCH_SCALE) != 0;
+        plan->chains.emplace_back();
+        rk::Chain& ch = plan->chains.back();
+        int rc = rk::build_chain(sh, lim, tb, ch);
+        if (rc != RK_OK) return rc;
+        // this item's workspace region: [ws_elems, ws_elems + ws_total)
+        const int64_t base0 = (int64_t)ws_elems, base1 = (int64_t)(ws_elems + ch.ws_elems);
+        for (size_t i = 0; i < ch.passes.size(); ++i) {
+            const rk::ChainPass& cp = ch.passes[i];
+            rk::QueuedPass qp;
+            qp.cp = &cp;
+            qp.a = cp.pp.args;
+            qp.item = k;
+            qp.level = (int)i;
+            qp.inverse = 0;
+            auto map = [&](int buf, rk_i64& off) -> rk_u32 {
+                if (buf == rk::BUF_IN) return RK_BATCH_BUF_IN;
+                if (buf == rk::BUF_OUT) return RK_BATCH_BUF_OUT;
+                off += buf == rk::BUF_WS0 ? base0 : base1;
+                return RK_BATCH_BUF_WS;
+            };
+            qp.src = map(cp.pp.src, qp.a.in_off);
+            qp.dst = map(cp.pp.dst, qp.a.out_off);
+            queued.push_back(qp);
         }
-        const float scale = (float)(1.0 / (double)it.length);
-        const bool norm_fwd = (it.flags & RK_BATCH_SCALE) != 0;
-        for (ItemPass& ip : chain) {
-            if (ip.last) {
-                ip.fwd_scale = norm_fwd ? scale : 1.0f;
-                ip.inv_scale = norm_fwd ? 1.0f : scale;
-            }
-            int rc = rk::build_tables(tb, ip.pp, ip.ap, it.length);
-            if (rc != RK_OK) return rc;
-            all.push_back(ip);
-        }
-        plan->n_items += it.count;
+        ws_elems += (ch.ws_total() + 31) & ~size_t(31);   // 256-byte aligned regions
     }
-    for (const auto& c : plan->copies) plan->n_items += (int64_t)c.total;
-
-    // queue order: level by level; inside a level the largest tiles first
-    // (longest-processing-time first), or item order
-    std::vector<size_t> order(all.size());
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
-        const ItemPass &x = all[a], &y = all[b];
-        if (x.level != y.level) return x.level < y.level;
-        if (o.order == 1) return x.item < y.item;
-        const int64_t tx = (int64_t)x.pp.C * x.pp.L, ty = (int64_t)y.pp.C * y.pp.L;
-        if (tx != ty) return tx > ty;
-        return x.item < y.item;
-    });
-    std::vector<int> pos(all.size());
-    for (size_t q = 0; q < order.size(); ++q) pos[order[q]] = (int)q;
 
     uint64_t total = 0;
-    plan->passes.resize(all.size());
-    for (size_t q = 0; q < order.size(); ++q) {
-        const size_t i = order[q];
-        const ItemPass& ip = all[i];
-        rk_batch_pass& bp = plan->passes[q];
-        memset(&bp, 0, sizeof(bp));
-        bp.a = ip.pp.args;
-        bp.first = total;
-        bp.kcase = (rk_u32)ip.kcase;
-        bp.src = ip.pp.src == rk::BUF_IN ? RK_BATCH_BUF_IN : RK_BATCH_BUF_WS;
-        bp.dst = ip.pp.dst == rk::BUF_OUT ? RK_BATCH_BUF_OUT : RK_BATCH_BUF_WS;
-        bp.wait = ip.first ? -1 : pos[i - 1];   // the item's previous pass
-        bp.wait_tiles = ip.first ? 0u : (rk_u32)all[i - 1].pp.args.n_tiles;
-        bp.signal = ip.last ? 0u : 1u;
-        bp.item_first = ip.first ? 1u : 0u;
-        bp.item_last = ip.last ? 1u : 0u;
-        bp.fwd_scale = ip.fwd_scale;
-        bp.inv_scale = ip.inv_scale;
-        total += ip.pp.args.n_tiles;
-        plan->levels = std::max(plan->levels, ip.level + 1);
-    }
-    if (total >= (uint64_t(1) << 31)) return fail(RK_ERR_UNSUPPORTED, "batch has too many tiles (%llu)", (unsigned long long)total);
-    plan->tile_pass.resize(total);
-    for (size_t q = 0; q < plan->passes.size(); ++q) {
-        const rk_batch_pass& bp = plan->passes[q];
-        for (rk_u64 t = 0; t < bp.a.n_tiles; ++t) plan->tile_pass[bp.first + t] = (rk_u32)q;
-    }
+    for (const auto& qp : queued) total += qp.a.n_tiles;
+    plan->flavour = rk::BATCH_LIGHT;
+    for (const auto& ch : plan->chains)
+        if (ch.blue) plan->flavour = rk::BATCH_BLUE;
+    if (total >= (uint64_t(1) << 31))
+        return fail(RK_ERR_UNSUPPORTED, "batch has too many tiles (%llu)", (unsigned long long)total);
     plan->ws_bytes = ws_elems * sizeof(float2);
     plan->launches = (total ? 1 : 0) + (int)plan->copies.size();
-
     memset(&plan->work, 0, sizeof(plan->work));
-    plan->work.n_passes = (rk_u32)plan->passes.size();
+    plan->work.n_passes = (rk_u32)queued.size();
     plan->work.total = (rk_u32)total;
     plan->work.spin_limit = kSpinLimit;
-    if (plan->dry) return RK_OK;
+    if (plan->dry) {
+        rk::assemble_queue(queued, plan->opt.order, plan->flavour, plan->passes, plan->tile_pass, plan->levels);
+        return RK_OK;
+    }
 
-    // one device allocation: tables | passes | tile_pass | done | state
+    // one device allocation: tables | passes | tile_pass | done | seen | state
     auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
     const size_t off_pass = align(tb.bytes.size());
-    const size_t off_tp = off_pass + align(plan->passes.size() * sizeof(rk_batch_pass));
-    const size_t off_done = off_tp + align(plan->tile_pass.size() * sizeof(rk_u32));
-    const size_t off_state = off_done + align(plan->passes.size() * sizeof(rk_u32));
+    const size_t off_tp = off_pass + align(queued.size() * sizeof(rk_batch_pass));
+    const size_t off_done = off_tp + align((size_t)total * sizeof(rk_u32));
+    const size_t off_seen = off_done + align(queued.size() * sizeof(rk_u32));
+    const size_t off_state = off_seen + align(queued.size() * sizeof(rk_u32));
     const size_t bytes = off_state + 256;
     RK_HIP(hipMalloc(&plan->dev, bytes));
     char* base = (char*)plan->dev;
-    // table pointers (build_tables recorded offsets into the table block)
-    for (size_t q = 0; q < order.size(); ++q) {
-        const rk::PassPlan& pp = all[order[q]].pp;
-        rk_pass_args& a = plan->passes[q].a;
-        auto tab = [&](int64_t off) -> const void* { return off < 0 ? nullptr : base + off; };
-        a.twl = (const float2*)tab(pp.twl_off);
-        a.tw_hi = (const double2*)tab(pp.twhi_off);
-        a.tw_lo = (const double2*)tab(pp.twlo_off);
-        a.chirp = (const float2*)tab(pp.chirp_off);
-        a.bspec = (const float2*)tab(pp.bspec_off);
-        a.tile_base = 0;
-        a.xcd_group = 0;
-    }
+    for (auto& qp : queued) rk::resolve_tables(qp.a, qp.cp->pp, base);
+    rk::assemble_queue(queued, plan->opt.order, plan->flavour, plan->passes, plan->tile_pass, plan->levels);
     if (!tb.bytes.empty()) RK_HIP(hipMemcpy(base, tb.bytes.data(), tb.bytes.size(), hipMemcpyHostToDevice));
     if (!plan->passes.empty())
         RK_HIP(hipMemcpy(base + off_pass, plan->passes.data(), plan->passes.size() * sizeof(rk_batch_pass),
@@ -307,9 +374,13 @@ static int create_queue(const rk_batch_item* items, int n_items, rk_batch_plan* 
         RK_HIP(hipMemcpy(base + off_tp, plan->tile_pass.data(), plan->tile_pass.size() * sizeof(rk_u32),
                          hipMemcpyHostToDevice));
     RK_HIP(hipMemset(base + off_done, 0, bytes - off_done));
+    // the plan executes on callers' streams, which do not wait for the null
+    // stream: the uploads above must have landed before this returns
+    RK_HIP(hipDeviceSynchronize());
     plan->work.passes = (const rk_batch_pass*)(base + off_pass);
     plan->work.tile_pass = (const rk_u32*)(base + off_tp);
     plan->work.done = (rk_u32*)(base + off_done);
+    plan->work.seen = (rk_u32*)(base + off_seen);
     plan->work.state = (rk_u32*)(base + off_state);
     return RK_OK;
 }
@@ -382,7 +453,7 @@ extern "C" int rk_batch_execute(const rk_batch_plan* plan, void* out, const void
     w.ws = (float2*)workspace;
     w.inverse = inverse ? 1u : 0u;
     void* params[] = {&w};
-    RK_HIP(hipLaunchKernel(rk::batch_kernel(), dim3(w.total), dim3(RK_BATCH_THREADS), params, 0, s));
+    RK_HIP(hipLaunchKernel(rk::batch_kernel(plan->flavour), dim3(w.total), dim3(RK_BATCH_THREADS), params, 0, s));
     return RK_OK;
 }
 
@@ -403,7 +474,8 @@ extern "C" int rk_batch_plan_info(const rk_batch_plan* plan, rk_batch_info* info
     info->levels = plan->levels;
     info->launches = plan->launches;
     info->mode = (int32_t)plan->opt.mode;
-    info->lds_bytes = rk::batch_lds_bytes();
+    info->lds_bytes = rk::batch_lds_bytes(plan->flavour);
+    info->flavour = plan->flavour;
     return RK_OK;
 }
 

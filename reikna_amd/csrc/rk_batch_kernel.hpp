@@ -3,7 +3,10 @@
 //
 // No reference counterpart: reikna compiles and launches one computation per
 // shape (reikna/fft/fft.py:723-756).  The queue protocol is described in
-// fft_batch.h; the grid has one work-group per queue position.  All data
+// fft_batch.h; the grid has one work-group per queue position and no atomic
+// is shared by the whole grid (same-address device atomics serialise at
+// ~11 ns each on MI355X: 94 us for one per work-group at 8192 work-groups).
+// All data
 // accesses of the kernel are device-coherent (CoherentIO), so a pass hands its
 // output to the next pass of the same transform through a tile counter alone:
 // agent-scope fences would write back or invalidate a whole XCD's L2 per tile.  (A
@@ -17,9 +20,14 @@
 // (batch_cases.inc): every length 2^k, 3*2^k and 5*2^k up to 4096 as a single
 // pass and as single-kernel Bluestein, and the pass lengths up to 256 of
 // multi-pass chains.
+//
+// This file is the kernel body; rk_batch_light.hip and rk_batch_blue.hip
+// instantiate it with two case lists (without / with the single-kernel
+// Bluestein cases, which need ~250 VGPRs and so lower the occupancy of every
+// case), defining RK_BATCH_CASES, RK_BATCH_KERNEL (the kernel's name) and
+// RK_BATCH_HOST (the namespace of its host-side case table).
 #include "fft_device.hpp"
 #include "fft_batch.h"
-#include "batch_cases.inc"
 
 namespace {
 
@@ -58,63 +66,53 @@ __device__ __forceinline__ void run_case(rk_u32 kc, const rk_pass_args& a, float
 
 }  // namespace
 
-extern "C" __global__ void __launch_bounds__(kThreads) rk_fft_batch(rk_batch_work w) {
+extern "C" __global__ void __launch_bounds__(kThreads) RK_BATCH_KERNEL(rk_batch_work w) {
     __shared__ float2 lds[(kLdsBytes + 7) / 8];
-    __shared__ rk_u32 s_item, s_last;
     const int tid = threadIdx.x;
-    // one queue position per work-group, taken in the order work-groups start
-    if (tid == 0) s_item = __hip_atomic_fetch_add(&w.state[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const rk_u32 item = __builtin_amdgcn_readfirstlane(s_item);
-    if (item < w.total) {
-        const rk_u32 p = __builtin_amdgcn_readfirstlane(w.tile_pass[item]);
-        const rk_batch_pass* bp = w.passes + p;
-        const rk_i32 wait = bp->wait;
-        if (wait >= 0) {
-            // the producing pass's tiles were all taken earlier from the queue,
-            // by work-groups that are running
-            if (tid == 0) {
-                const rk_u32 need = bp->wait_tiles;
-                rk_u32 polls = 0;
-                while (__hip_atomic_load(&w.done[wait], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
-                    if (++polls > w.spin_limit) {
-                        __hip_atomic_store(&w.state[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(4);
-                }
-            }
-            __syncthreads();
-        }
-        rk_pass_args a = bp->a;
-        const rk_u32 src = bp->src, dst = bp->dst;
-        a.in = src == RK_BATCH_BUF_IN ? w.in : src == RK_BATCH_BUF_OUT ? w.out : w.ws;
-        a.out = dst == RK_BATCH_BUF_OUT ? w.out : w.ws;
-        const bool inv = w.inverse != 0;
-        a.in_sign = (inv && bp->item_first) ? -1.0f : 1.0f;
-        a.out_sign = (inv && bp->item_last) ? -1.0f : 1.0f;
-        a.out_scale = inv ? bp->inv_scale : bp->fwd_scale;
-        run_case(bp->kcase, a, lds, (rk_u64)(item - bp->first), tid);
-        if (bp->signal) {
-            // every store of the tile has completed (they are device-coherent)
-            // before the tile is counted
-            __builtin_amdgcn_s_waitcnt(0);
-            __syncthreads();
-            if (tid == 0) __hip_atomic_fetch_add(&w.done[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
-    // the last work-group to finish clears the queue for the next launch
-    if (tid == 0) {
-        const rk_u32 prev = __hip_atomic_fetch_add(&w.state[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (prev + 1 == gridDim.x) ? 1u : 0u;
-    }
-    __syncthreads();
-    if (s_last) {
-        for (rk_u32 i = tid; i < w.n_passes; i += kThreads) w.done[i] = 0u;
+    // queue position = work-group index: work-groups are dispatched in index
+    // order, so every earlier position is running or finished
+    const rk_u32 item = blockIdx.x;
+    if (item >= w.total) return;
+    const rk_u32 p = __builtin_amdgcn_readfirstlane(w.tile_pass[item]);
+    const rk_batch_pass* bp = w.passes + p;
+    const rk_i32 wait = bp->wait;
+    if (wait >= 0) {
         if (tid == 0) {
-            w.state[0] = 0u;
-            w.state[1] = 0u;
+            const rk_u32 need = bp->wait_tiles;
+            rk_u32 polls = 0;
+            while (__hip_atomic_load(&w.done[wait], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+                if (++polls > w.spin_limit) {
+                    __hip_atomic_store(&w.state[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(4);
+            }
+            // the last tile of this pass to get past the wait clears the
+            // producer's counters for the next launch (every tile of this
+            // pass has seen the count by then)
+            const rk_u32 seen = __hip_atomic_fetch_add(&w.seen[wait], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (seen + 1 == (rk_u32)bp->a.n_tiles) {
+                __hip_atomic_store(&w.done[wait], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&w.seen[wait], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
+        __syncthreads();
+    }
+    rk_pass_args a = bp->a;
+    const rk_u32 src = bp->src, dst = bp->dst;
+    a.in = src == RK_BATCH_BUF_IN ? w.in : src == RK_BATCH_BUF_OUT ? w.out : src == RK_BATCH_BUF_WS ? w.ws : a.in;
+    a.out = dst == RK_BATCH_BUF_OUT ? w.out : dst == RK_BATCH_BUF_WS ? w.ws : a.out;
+    const bool inv = (w.inverse | bp->inverse) != 0;
+    a.in_sign = (inv && bp->item_first) ? -1.0f : 1.0f;
+    a.out_sign = (inv && bp->item_last) ? -1.0f : 1.0f;
+    a.out_scale = inv ? bp->inv_scale : bp->fwd_scale;
+    run_case(bp->kcase, a, lds, (rk_u64)(item - bp->first), tid);
+    if (bp->signal) {
+        // every store of the tile has completed (they are device-coherent)
+        // before the tile is counted
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid == 0) __hip_atomic_fetch_add(&w.done[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -122,10 +120,11 @@ extern "C" __global__ void __launch_bounds__(kThreads) rk_fft_batch(rk_batch_wor
 #include "rk_internal.hpp"
 
 namespace rk {
+namespace RK_BATCH_HOST {
 
-const void* batch_kernel() { return (const void*)&rk_fft_batch; }
+const void* kernel() { return (const void*)&RK_BATCH_KERNEL; }
 
-const std::vector<BatchCase>& batch_cases() {
+const std::vector<BatchCase>& cases() {
     static const std::vector<BatchCase> all = [] {
         std::vector<BatchCase> v;
 #define RK_CASE_ENTRY(id, L, C, V) v.push_back(BatchCase{id, L, C, V, rkfft::pass_threads<L, C>(), rkfft::tile_lds_bytes<L, C>()});
@@ -136,6 +135,7 @@ const std::vector<BatchCase>& batch_cases() {
     return all;
 }
 
-int batch_lds_bytes() { return kLdsBytes; }
+int lds_bytes() { return kLdsBytes; }
 
+}  // namespace RK_BATCH_HOST
 }  // namespace rk

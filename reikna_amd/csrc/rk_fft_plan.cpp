@@ -90,6 +90,7 @@ ChainLimits default_limits() {
     l.pow2_bluestein = false;
     l.blue_single_max = l.single_max;
     l.length_ok = nullptr;
+    l.strided_single_max = 0;
     return l;
 }
 
@@ -326,9 +327,12 @@ int axis_passes(int64_t n, uint32_t flags, const ChainLimits& lim, std::vector<A
         out.push_back(ap);
         return RK_OK;
     }
-    if (!split_factors(m, lim, f))
+    if (m <= kSingle && length_ok(lim, m) && !(flags & RK_FFT_FORCE_MULTIPASS)) {
+        f.assign(1, m);   // two single-pass chains
+    } else if (!split_factors(m, lim, f)) {
         return fail(RK_ERR_UNSUPPORTED, "cannot split Bluestein length %lld", (long long)m);
-    if (f.size() == 1) {   // forced multi-pass of a short M
+    }
+    if (f.size() == 1 && (flags & RK_FFT_FORCE_MULTIPASS)) {   // forced multi-pass of a short M
         int64_t d = 0;
         for (int64_t k = 2; k * k <= m; ++k)
             if (m % k == 0) d = k;
@@ -559,6 +563,62 @@ int build_tables(TableBuilder& tb, PassPlan& pp, const AxisPass& ap, int64_t nre
     return RK_OK;
 }
 
+int route_passes(const rk_fft_desc& d, const std::vector<int>& work, const Layout& lin, const Layout& lout,
+                 const ChainLimits& lim, TileWidthFn tile_width, const PassHook& hook, std::vector<PassPlan>& passes,
+                 size_t& ws_elems, bool& two_ws) {
+    int cur = BUF_IN;
+    int64_t cur_shape[RK_MAX_DIMS];
+    for (int k = 0; k < d.ndim; ++k) cur_shape[k] = d.shape[k];
+    for (size_t wi = 0; wi < work.size(); ++wi) {
+        const int axis = work[wi];
+        const int64_t n = d.shape[axis];
+        std::vector<AxisPass> aps;
+        ChainLimits al = lim;
+        if (lim.strided_single_max > 0 && lin.strides[axis] != 1)
+            al.single_max = std::min(al.single_max, lim.strided_single_max);
+        int rc = axis_passes(n, d.flags, al, aps);
+        if (rc != RK_OK) return rc;
+        if (aps.size() >= 3) two_ws = true;
+        int ws_next = BUF_WS0;
+        for (size_t pi = 0; pi < aps.size(); ++pi) {
+            const AxisPass& ap = aps[pi];
+            const bool axis_last = (pi + 1 == aps.size());
+            int dst;
+            const bool plan_last = axis_last && (wi + 1 == work.size());
+            if (axis_last && (plan_last || !(d.flags & RK_FFT_OUT_LAST_ONLY))) {
+                dst = BUF_OUT;
+            } else {
+                dst = ws_next;
+                if (dst == cur) dst = (dst == BUF_WS0) ? BUF_WS1 : BUF_WS0;
+                ws_next = (dst == BUF_WS0) ? BUF_WS1 : BUF_WS0;
+            }
+            // layouts of src and dst for this pass
+            int64_t s_shape[RK_MAX_DIMS], d_shape[RK_MAX_DIMS];
+            for (int k = 0; k < d.ndim; ++k) s_shape[k] = d_shape[k] = cur_shape[k];
+            s_shape[axis] = ap.in_len;
+            d_shape[axis] = ap.out_len;
+            Layout ls = (cur == BUF_IN) ? lin : (cur == BUF_OUT) ? lout : contiguous_layout(d.ndim, s_shape);
+            Layout ld = (dst == BUF_OUT) ? lout : contiguous_layout(d.ndim, d_shape);
+            if (dst == BUF_WS0 || dst == BUF_WS1) {
+                size_t e = 1;
+                for (int k = 0; k < d.ndim; ++k) e *= (size_t)d_shape[k];
+                ws_elems = std::max(ws_elems, e);
+            }
+            PassPlan pp;
+            pp.src = cur;
+            pp.dst = dst;
+            rc = describe_pass(ap, ls, ld, axis, tile_width, pp);
+            if (rc != RK_OK) return rc;
+            rc = hook(pp, ap, n);
+            if (rc != RK_OK) return rc;
+            passes.push_back(pp);
+            cur = dst;
+            cur_shape[axis] = ap.out_len;
+        }
+    }
+    return RK_OK;
+}
+
 static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
     if (!d || !out) return fail(RK_ERR_INVALID, "rk_fft_plan_create: null argument");
     *out = nullptr;
@@ -622,62 +682,17 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
     }
 
     TableBuilder tb;
-    int cur = BUF_IN;
-    int64_t cur_shape[RK_MAX_DIMS];
-    for (int k = 0; k < d->ndim; ++k) cur_shape[k] = d->shape[k];
     size_t ws_elems = 0;
     bool two_ws = false;
-
-    for (size_t wi = 0; wi < work.size(); ++wi) {
-        const int axis = work[wi];
-        const int64_t n = d->shape[axis];
-        std::vector<AxisPass> aps;
-        int rc = axis_passes(n, d->flags, default_limits(), aps);
+    PassHook hook = [&](PassPlan& pp, const AxisPass& ap, int64_t n) -> int {
+        int rc = find_kernel(pp, d->flags);
         if (rc != RK_OK) return rc;
-        if (aps.size() >= 3) two_ws = true;
-        int ws_next = BUF_WS0;
-        for (size_t pi = 0; pi < aps.size(); ++pi) {
-            const AxisPass& ap = aps[pi];
-            const bool axis_last = (pi + 1 == aps.size());
-            int dst;
-            const bool plan_last = axis_last && (wi + 1 == work.size());
-            if (axis_last && (plan_last || !(d->flags & RK_FFT_OUT_LAST_ONLY))) {
-                dst = BUF_OUT;
-            } else {
-                dst = ws_next;
-                if (dst == cur) dst = (dst == BUF_WS0) ? BUF_WS1 : BUF_WS0;
-                ws_next = (dst == BUF_WS0) ? BUF_WS1 : BUF_WS0;
-            }
-            // layouts of src and dst for this pass
-            int64_t s_shape[RK_MAX_DIMS], d_shape[RK_MAX_DIMS];
-            for (int k = 0; k < d->ndim; ++k) s_shape[k] = d_shape[k] = cur_shape[k];
-            s_shape[axis] = ap.in_len;
-            d_shape[axis] = ap.out_len;
-            Layout ls = (cur == BUF_IN) ? lin : (cur == BUF_OUT) ? lout : contiguous_layout(d->ndim, s_shape);
-            Layout ld = (dst == BUF_OUT) ? lout : contiguous_layout(d->ndim, d_shape);
-            if (dst == BUF_WS0 || dst == BUF_WS1) {
-                size_t e = 1;
-                for (int k = 0; k < d->ndim; ++k) e *= (size_t)d_shape[k];
-                ws_elems = std::max(ws_elems, e);
-            }
-
-            PassPlan pp;
-            pp.src = cur;
-            pp.dst = dst;
-            rc = describe_pass(ap, ls, ld, axis, plan_tile_width, pp);
-            if (rc != RK_OK) return rc;
-
-            rc = find_kernel(pp, d->flags);
-            if (rc != RK_OK) return rc;
-            const uint64_t max_grid = (uint64_t(1) << 31) / (uint64_t)pp.threads;
-            pp.grid = std::min<uint64_t>(pp.args.n_tiles, max_grid);
-            rc = build_tables(tb, pp, ap, n);
-            if (rc != RK_OK) return rc;
-            plan->passes.push_back(pp);
-            cur = dst;
-            cur_shape[axis] = ap.out_len;
-        }
-    }
+        const uint64_t max_grid = (uint64_t(1) << 31) / (uint64_t)pp.threads;
+        pp.grid = std::min<uint64_t>(pp.args.n_tiles, max_grid);
+        return build_tables(tb, pp, ap, n);
+    };
+    int rc = route_passes(*d, work, lin, lout, default_limits(), plan_tile_width, hook, plan->passes, ws_elems, two_ws);
+    if (rc != RK_OK) return rc;
     plan->passes.front().is_first = true;
     plan->passes.back().is_last = true;
 
@@ -717,6 +732,9 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
     if (!tb.bytes.empty() && !(d->flags & RK_FFT_DRY_RUN)) {
         RK_HIP(hipMalloc(&plan->tables, tb.bytes.size()));
         RK_HIP(hipMemcpy(plan->tables, tb.bytes.data(), tb.bytes.size(), hipMemcpyHostToDevice));
+        // the plan executes on callers' streams, which do not wait for the
+        // null stream: the tables must have landed before this returns
+        RK_HIP(hipDeviceSynchronize());
     }
     *out = plan.release();
     return RK_OK;

@@ -6,6 +6,7 @@
 
 #include <cstdarg>
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -96,6 +97,7 @@ struct ChainLimits {
     int64_t blue_single_max;   // longest single-kernel Bluestein length M
     bool pow2_bluestein;   // Bluestein lengths are powers of two
     bool (*length_ok)(int64_t L);   // pass lengths a kernel exists for (null: any smooth length)
+    int64_t strided_single_max;     // single_max of axes that are not contiguous in the input (0: single_max)
 };
 ChainLimits default_limits();
 
@@ -135,17 +137,45 @@ typedef int (*TileWidthFn)(int L, bool colmap);
 int describe_pass(const AxisPass& ap, const Layout& ls, const Layout& ld, int axis, TileWidthFn tile_width,
                   PassPlan& pp);
 
+// Called for every pass route_passes builds, after describe_pass: picks the
+// kernel and builds the tables (n: the axis length before Bluestein padding).
+typedef std::function<int(PassPlan& pp, const AxisPass& ap, int64_t n)> PassHook;
+
+// The passes of a transform over `work` (axes innermost first) from layout
+// lin to lout: the first pass reads BUF_IN, each axis's last pass writes
+// BUF_OUT (or only the very last one with RK_FFT_OUT_LAST_ONLY), passes in
+// between ping-pong through BUF_WS0 / BUF_WS1, contiguous buffers of
+// ws_elems elements each (two_ws: both are needed).
+int route_passes(const rk_fft_desc& d, const std::vector<int>& work, const Layout& lin, const Layout& lout,
+                 const ChainLimits& lim, TileWidthFn tile_width, const PassHook& hook, std::vector<PassPlan>& passes,
+                 size_t& ws_elems, bool& two_ws);
+
 // Stage twiddles, inter-pass twiddles, chirp and Bluestein spectrum of a pass
 // (nreal: the transform length before Bluestein padding).
 int build_tables(TableBuilder& tb, PassPlan& pp, const AxisPass& ap, int64_t nreal);
 
-// Tile cases of the persistent batch kernel (rk_batch_kernel.hip).
+// Tile cases of the batch kernel (rk_batch_kernel.hpp).
 struct BatchCase {
     int id, L, C, variant, threads, lds_bytes;
 };
-const std::vector<BatchCase>& batch_cases();
-const void* batch_kernel();
-int batch_lds_bytes();
+// Two flavours of the kernel: light (no single-kernel Bluestein case) and
+// blue (a superset with them); a batch uses blue only if it needs one.
+enum BatchFlavour { BATCH_LIGHT = 0, BATCH_BLUE = 1 };
+namespace batch_light {
+const void* kernel();
+const std::vector<BatchCase>& cases();
+int lds_bytes();
+}  // namespace batch_light
+namespace batch_blue {
+const void* kernel();
+const std::vector<BatchCase>& cases();
+int lds_bytes();
+}  // namespace batch_blue
+inline const void* batch_kernel(int flavour) { return flavour ? batch_blue::kernel() : batch_light::kernel(); }
+inline const std::vector<BatchCase>& batch_cases(int flavour) {
+    return flavour ? batch_blue::cases() : batch_light::cases();
+}
+inline int batch_lds_bytes(int flavour) { return flavour ? batch_blue::lds_bytes() : batch_light::lds_bytes(); }
 
 // Strided N-d copy (trivial plans: no axis longer than 1).
 struct CopyArgs {

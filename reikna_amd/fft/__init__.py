@@ -2,6 +2,7 @@
 
 from .batch import BatchFFT, BatchItem, BatchOptions
 from .fft import FFT
+from .fftshift import FFTShift, fftshift_transformation
 from .native import NativeFftPlan
 
-__all__ = ["FFT", "BatchFFT", "BatchItem", "BatchOptions", "NativeFftPlan"]
+__all__ = ["FFT", "FFTShift", "BatchFFT", "BatchItem", "BatchOptions", "NativeFftPlan", "fftshift_transformation"]

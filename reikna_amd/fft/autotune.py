@@ -30,9 +30,8 @@ def default_candidates() -> list[BatchOptions]:
     out = []
     for single in (1024, 2048, 4096):
         for factor in (64, 128, 256):
-            for blue in (1024, 2048):
-                for order in (0, 1):
-                    out.append(BatchOptions(single_max=single, factor_max=factor, blue_max=blue, order=order))
+            for order in (0, 2):
+                out.append(BatchOptions(single_max=single, factor_max=factor, order=order))
     return out
 
 

@@ -48,8 +48,8 @@ class BatchOptions:
     mode: str = "queue"      # "queue" (one launch) or "grouped" (one plan per item)
     single_max: int = 0      # longest single-pass length (<= 4096)
     factor_max: int = 0      # longest pass of a multi-pass chain (<= 256)
-    blue_max: int = 0        # longest single-kernel Bluestein length (<= 2048)
-    order: int = 0           # queue order in a level: 0 largest tiles first, 1 item order
+    blue_max: int = 0        # reserved (Bluestein runs as two chains in the batch kernel)
+    order: int = 0           # queue order: 0 critical path, 1 levels in item order, 2 levels, largest tiles first
 
     def to_c(self, dry: bool = False) -> _lib.BatchOptions:
         if self.mode not in ("queue", "grouped"):

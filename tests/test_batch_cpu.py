@@ -42,7 +42,7 @@ def test_lengths_without_a_case_use_bluestein(vdev):
 
 def test_knobs_change_the_chains(vdev):
     assert info(vdev, [4096], single_max=1024)["levels"] == 2
-    assert info(vdev, [997], blue_max=1024)["levels"] == 4
+    assert info(vdev, [2003], single_max=2048)["levels"] == 4   # Bluestein over 4096 in chains of two passes
     assert info(vdev, [65536], factor_max=64)["levels"] == 3
 
 
@@ -103,3 +103,18 @@ def test_sample_lengths_families():
             fam["prime"] += 1
     assert all(v > 64 for v in fam.values())
     assert ls == sample_lengths(512, seed=5)
+
+
+def test_kernel_flavour(vdev):
+    # the light kernel unless a pass needs a single-kernel Bluestein case
+    assert info(vdev, [64, 8192, 3 * 2**10])["flavour"] == 0
+    assert info(vdev, [64, 997])["flavour"] == 1
+    assert info(vdev, [65521])["flavour"] == 0          # Bluestein over 2^17 runs as chains
+    assert info(vdev, [2003])["flavour"] == 0           # Bluestein over 4096: two single-pass chains
+
+
+def test_order_modes_keep_dependencies(vdev):
+    lengths = sample_lengths(64, seed=3)
+    for order in (0, 1, 2):
+        i = info(vdev, lengths, order=order)
+        assert i["launches"] == 1 and i["levels"] >= 4

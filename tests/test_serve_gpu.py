@@ -1,0 +1,134 @@
+"""GPU tests of the request server (continuous batching of FFT requests).
+
+Every completed request is checked against numpy.fft in FP64 at the
+complex64 bound; a request computed alone and inside a mixed stream gives
+bit-identical results (its pass chain depends only on its shape); the HBM
+budget bounds what is admitted at any time.
+"""
+
+import math
+
+import numpy
+import pytest
+
+import oracle
+from reikna_amd._lib import BudgetError
+from reikna_amd.fft.serve import FFTServer, RequestSpec, make_specs, run_stream, sample_shapes
+
+pytestmark = pytest.mark.gpu
+
+GiB = 2**30
+
+
+def reference(spec: RequestSpec, host_in: numpy.ndarray) -> numpy.ndarray:
+    x = host_in[: spec.elements].reshape(spec.count, *spec.shape)
+    axes = tuple(range(1, 1 + len(spec.shape)))
+    return oracle.ifftn(x, axes) if spec.inverse else oracle.fftn(x, axes)
+
+
+def check_output(server, queue, c, spec, cache):
+    key = id(spec.input)
+    if key not in cache:
+        cache[key] = spec.input.get(queue)
+    out = server.output(c, spec).get(queue)
+    ref = reference(spec, cache[key])
+    err = oracle.rel_l2(out, ref)
+    tol = oracle.tolerance(spec.shape)
+    assert err <= tol, f"request {c.id} {spec.shape} x {spec.count}: rel-L2 {err:.3g} > {tol:.3g}"
+
+
+def drain(server, specs, queue=None, check=True, keep=()):
+    server.submit([(i, s, 0) for i, s in enumerate(specs)])
+    done, cache, outs = 0, {}, {}
+    while done < len(specs):
+        for c in server.progress():
+            done += 1
+            if check:
+                check_output(server, queue, c, specs[c.id], cache)
+            if c.id in keep:
+                outs[c.id] = server.output(c, specs[c.id]).get(queue)
+            server.release(c.id)
+    return outs
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+def test_mixed_requests_match_oracle(queue, mode):
+    shapes = sample_shapes(40, seed=17, one_d=(8, 16), two_d=(7, 9), max_count=4)
+    shapes += [((997,), 2), ((65536,), 1), ((96, 160), 1), ((1000,), 3)]
+    specs = make_specs(queue, shapes, seed=18)
+    server = FFTServer(queue.device, budget_bytes=1 * GiB, slots=3, mode=mode)
+    drain(server, specs, queue)
+    st = server.stats
+    assert st["completed"] == len(specs) and st["pending"] == 0 and st["budget_used"] == 0
+    if mode == "batched":
+        assert st["launches"] < len(specs)   # requests shared launches
+    server.close()
+
+
+def test_inverse_requests(queue):
+    shapes = [((1024,), 2), ((128, 128), 1), ((5003,), 1)]
+    specs = make_specs(queue, shapes, seed=19, inverse=True)
+    server = FFTServer(queue.device, budget_bytes=256 * 2**20)
+    drain(server, specs, queue)
+    server.close()
+
+
+def test_alone_and_in_mix_identical(queue):
+    shapes = sample_shapes(48, seed=20, one_d=(8, 17), two_d=(7, 10), max_count=8)
+    specs = make_specs(queue, shapes, seed=21)
+    keep = set(range(0, len(specs), 5))
+    server = FFTServer(queue.device, budget_bytes=2 * GiB, slots=4)
+    res = run_stream(server, specs, rate=20000.0, seed=22, keep=keep,
+                     on_complete=lambda c, s: None)
+    mixed = {}
+    for c in res.completions:
+        if c.id in keep:
+            mixed[c.id] = server.output(c, specs[c.id]).get(queue)
+            server.release(c.id)
+    server.close()
+    for rid in sorted(keep):
+        alone = FFTServer(queue.device, budget_bytes=2 * GiB, slots=1)
+        out = drain(alone, [specs[rid]], queue, check=True, keep={0})[0]
+        alone.close()
+        assert numpy.array_equal(out, mixed[rid]), f"request {rid} {specs[rid].shape} x {specs[rid].count}"
+
+
+def test_budget_bounds_admission(queue):
+    shapes = [((4096,), 16)] * 12          # 512 KiB output each
+    specs = make_specs(queue, shapes, seed=23)
+    budget = 3 * 512 * 1024 + 4096         # three requests' outputs at a time
+    server = FFTServer(queue.device, budget_bytes=budget, slots=4)
+    server.submit([(i, s, 0) for i, s in enumerate(specs)])
+    done, peak, cache = 0, 0, {}
+    while done < len(specs):
+        comps = server.progress()
+        st = server.stats
+        peak = max(peak, st["budget_used"])
+        assert st["budget_used"] <= budget
+        for c in comps:
+            check_output(server, queue, c, specs[c.id], cache)
+            server.release(c.id)
+            done += 1
+    assert peak > 0
+    with pytest.raises(BudgetError):
+        server.submit([(99, make_specs(queue, [((65536,), 8)], seed=24)[0], 0)])
+    server.close()
+
+
+def test_large_requests_run_plain(queue):
+    shapes = [((2**18,), 4), ((512,), 3), ((256, 256), 2), ((2**20,), 1)]
+    specs = make_specs(queue, shapes, seed=25)
+    server = FFTServer(queue.device, budget_bytes=1 * GiB, slots=2, plain_min_elements=2**20)
+    drain(server, specs, queue)
+    server.close()
+
+
+def test_stream_reports_latency(queue):
+    shapes = sample_shapes(60, seed=26, one_d=(8, 14), two_d=(7, 8), max_count=4)
+    specs = make_specs(queue, shapes, seed=27)
+    server = FFTServer(queue.device, budget_bytes=1 * GiB)
+    res = run_stream(server, specs, rate=5000.0, seed=28)
+    assert res.completed == len(specs)
+    assert 0 < res.p50_ms <= res.p99_ms
+    assert res.throughput_rps > 0 and math.isfinite(res.gflops)
+    server.close()
