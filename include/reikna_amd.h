@@ -341,6 +341,12 @@ int rk_server_progress(rk_server* server, rk_completion* done, int max_done, int
 int rk_server_release(rk_server* server, uint64_t id);
 int rk_server_stats(const rk_server* server, rk_server_stats_t* stats);
 uint64_t rk_server_now_ns(void);
+/* Open-loop replay: submit each request when rk_server_now_ns() reaches its
+   t_arrival_ns, progress the server meanwhile, and write the n completions to
+   done (in completion order); outputs are released as they complete unless
+   keep_outputs.  Fails if nothing completes for stall_ms (0: 10 s). */
+int rk_server_replay(rk_server* server, const rk_request* requests, int n,
+                     rk_completion* done, int keep_outputs, int stall_ms);
 int rk_server_destroy(rk_server* server);
 
 /* ------------------------------------------------------------------ */

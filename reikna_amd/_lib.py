@@ -257,6 +257,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "rk_server_stats": (_I, [_P, ctypes.POINTER(ServerStats)]),
     "rk_server_now_ns": (ctypes.c_uint64, []),
     "rk_server_destroy": (_I, [_P]),
+    "rk_server_replay": (_I, [_P, ctypes.POINTER(Request), _I, ctypes.POINTER(Completion), _I, _I]),
     "rk_graph_begin": (_I, [_P]),
     "rk_graph_end": (_I, [_P, _PP]),
     "rk_graph_launch": (_I, [_P, _P]),

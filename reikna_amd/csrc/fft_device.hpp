@@ -358,13 +358,15 @@ struct PlainIO {
 // shared by all CUs).  Used where one kernel launch hands data from one
 // work-group to another: after the producer's stores have completed, a
 // counter in memory is enough to publish them, with no cache write-back or
-// invalidation in between.
+// invalidation in between.  (Choosing cached or coherent accesses per pass
+// with a run-time branch was measured: the second code path raised the
+// batch kernel from 160 to 200 VGPRs and cost more than it saved.)
 struct CoherentIO {
     static constexpr bool kBranchFree = true;
     __device__ __forceinline__ float2 load(const rk_pass_args& a, rk_u64 /*o*/, rk_u64 /*m*/, rk_u64 /*i*/,
                                            rk_i64 addr) const {
-        const rk_u64 bits = __hip_atomic_load(reinterpret_cast<rk_u64*>(const_cast<float2*>(a.in + addr)), __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT);
+        const rk_u64 bits = __hip_atomic_load(reinterpret_cast<rk_u64*>(const_cast<float2*>(a.in + addr)),
+                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return make_float2(__uint_as_float((unsigned)bits), __uint_as_float((unsigned)(bits >> 32)));
     }
     __device__ __forceinline__ void store(const rk_pass_args& a, rk_u64 /*o*/, rk_u64 /*k*/, rk_u64 /*i*/,
@@ -397,7 +399,11 @@ __device__ __forceinline__ float2 load_elem(const rk_pass_args& a, const IO& io,
     const bool ok = ic.valid && m < a.n_valid_in;
     float2 x = make_float2(0.0f, 0.0f);
     if constexpr (IO::kBranchFree) {
-        const float2 y = io.load(a, ic.o, m, ic.i, ok ? ic.in_base + (rk_i64)m * a.in_sa : 0);
+        // a lane with nothing to load reads its instance's first position (a
+        // valid address, different per instance) and discards it: one shared
+        // clamp address would be a hot spot for loads that bypass the caches
+        const rk_i64 spare = ic.in_base + (rk_i64)(ic.jpass < a.n_valid_in ? ic.jpass : 0) * a.in_sa;
+        const float2 y = io.load(a, ic.o, m, ic.i, ok ? ic.in_base + (rk_i64)m * a.in_sa : spare);
         x = ok ? y : x;
     } else {
         if (ok) x = io.load(a, ic.o, m, ic.i, ic.in_base + (rk_i64)m * a.in_sa);
@@ -623,7 +629,7 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
             float2 x = make_float2(0.0f, 0.0f);
             const bool ok = inst0 + ce < a.n_inst;
             if constexpr (IO::kBranchFree) {
-                const float2 y = io.load(a, inst0 + ce, p, 0, ok ? base + e : 0);
+                const float2 y = io.load(a, inst0 + ce, p, 0, ok ? base + e : base);
                 x = ok ? y : x;
             } else {
                 if (ok) x = io.load(a, inst0 + ce, p, 0, base + e);

@@ -6,10 +6,10 @@
 // fft_batch.h; the grid has one work-group per queue position and no atomic
 // is shared by the whole grid (same-address device atomics serialise at
 // ~11 ns each on MI355X: 94 us for one per work-group at 8192 work-groups).
-// All data
-// accesses of the kernel are device-coherent (CoherentIO), so a pass hands its
-// output to the next pass of the same transform through a tile counter alone:
-// agent-scope fences would write back or invalidate a whole XCD's L2 per tile.  (A
+// All data accesses of the kernel are device-coherent (CoherentIO), so a
+// pass hands its output to the next pass of the same transform through a
+// tile counter alone: agent-scope fences would write back or invalidate a
+// whole XCD's L2 per tile.  (A
 // grid-stride loop over the queue was measured to lose per-case register
 // allocation: loop-invariant code of all cases is hoisted in front of the
 // loop, and 33 cases already spill.)  Every tile case is one fft_pass_tile<L, C, V> instantiation

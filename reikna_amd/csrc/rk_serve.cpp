@@ -448,3 +448,43 @@ extern "C" int rk_server_stats(const rk_server* s, rk_server_stats_t* st) {
 }
 
 extern "C" uint64_t rk_server_now_ns(void) { return now_ns(); }
+
+// Open-loop replay of a request stream: each request is submitted when the
+// clock reaches its t_arrival_ns, the server is progressed in between, and
+// done[k] receives the k-th completion.  Outputs are released as they
+// complete unless keep_outputs.  Gives up (RK_ERR_HIP) if nothing completes
+// for `stall_ms` while requests are outstanding.
+extern "C" int rk_server_replay(rk_server* s, const rk_request* reqs, int n, rk_completion* done, int keep_outputs,
+                                int stall_ms) {
+    if (!s || n < 0 || (n > 0 && (!reqs || !done))) return fail(RK_ERR_INVALID, "rk_server_replay: bad arguments");
+    int next = 0, completed = 0;
+    std::vector<rk_completion> buf(1024);
+    uint64_t last_progress = now_ns();
+    const uint64_t stall_ns = (uint64_t)(stall_ms > 0 ? stall_ms : 10000) * 1000000ull;
+    while (completed < n) {
+        const uint64_t t = now_ns();
+        int k = next;
+        while (k < n && reqs[k].t_arrival_ns <= t) ++k;
+        if (k > next) {
+            int rc = rk_server_submit(s, reqs + next, k - next);
+            if (rc != RK_OK) return rc;
+            next = k;
+        }
+        int got = 0;
+        int rc = rk_server_progress(s, buf.data(), (int)buf.size(), &got);
+        if (rc != RK_OK) return rc;
+        for (int i = 0; i < got; ++i) {
+            done[completed++] = buf[i];
+            if (!keep_outputs) {
+                rc = rk_server_release(s, buf[i].id);
+                if (rc != RK_OK) return rc;
+            }
+        }
+        if (got) {
+            last_progress = t;
+        } else if (next > completed && t - last_progress > stall_ns) {
+            return fail(RK_ERR_HIP, "rk_server_replay: no request completed for %d ms", stall_ms > 0 ? stall_ms : 10000);
+        }
+    }
+    return RK_OK;
+}

@@ -185,12 +185,15 @@ def norm_const(arr_t: Any, order: float) -> Transformation:
         ],
         """
         ${output.ctype} _n = ${norm}(${input.load_same});
-        %if order != 2:
+        %if order == 1:
+        _n = sqrt(_n);
+        %elif order != 2:
         _n = ${powf}(_n, ${dtypes.c_constant(order / 2, output.dtype)});
         %endif
         ${output.store_same}(_n);
         """,
-        render_kwds=dict(norm=functions.norm(inp.dtype), order=order, powf="powf" if out_dtype == numpy.float32 else "pow"),
+        render_kwds=dict(norm=functions.norm(inp.dtype), order=order, dtypes=dtypes,
+                         powf="powf" if out_dtype == numpy.float32 else "pow"),
     )
 
 
