@@ -14,6 +14,7 @@
 //     (one launch per pass per item: the per-request path).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -27,7 +28,7 @@ namespace {
 constexpr int64_t kBatchSingleMax = 4096;
 constexpr int64_t kBatchFactorMax = 256;
 constexpr int64_t kBatchStridedMax = 256;   // column axes: >= 16 adjacent instances per tile
-constexpr uint32_t kSpinLimit = 1u << 24;   // ~2 s of 256-cycle sleeps
+constexpr uint32_t kSpinLimit = 1u << 24;   // polls: ~1 s at the shortest back-off
 
 const rk::BatchCase* find_case(int L, int variant, int flavour = rk::BATCH_BLUE) {
     for (const auto& c : rk::batch_cases(flavour))
@@ -140,8 +141,27 @@ int build_chain(const ItemShape& it, const ChainLimits& lim, TableBuilder& tb, C
     return rc;
 }
 
+rk_u32 batch_poll_sleep_max() {
+    static const rk_u32 v = [] {
+        const char* e = getenv("RK_BATCH_POLL_MAX");
+        const long x = e && *e ? atol(e) : 16;
+        return (rk_u32)std::min<long>(std::max<long>(x, 1), 1024);
+    }();
+    return v;
+}
+
+// Tile groups of a counted pass of n tiles: about sqrt(n), so that the
+// group counters and the finished-group counter see similar numbers of
+// atomics.
+static rk_u32 tile_groups(uint64_t n) {
+    rk_u32 g = 1;
+    while (g < RK_BATCH_GROUPS_MAX && (uint64_t)(g + 1) * (g + 1) <= n) ++g;
+    return g;
+}
+
 uint64_t assemble_queue(const std::vector<QueuedPass>& in, int order_mode, int flavour,
-                        std::vector<rk_batch_pass>& out, std::vector<rk_u32>& tile_pass, int& levels) {
+                        std::vector<rk_batch_pass>& out, std::vector<rk_u32>& tile_pass, int& levels,
+                        size_t& lines) {
     // chain length of every pass's item (the passes of an item are consecutive)
     std::vector<int> rank(in.size());
     int longest = 0;
@@ -172,6 +192,7 @@ uint64_t assemble_queue(const std::vector<QueuedPass>& in, int order_mode, int f
     for (size_t q = 0; q < order.size(); ++q) pos[order[q]] = (int)q;
     uint64_t total = 0;
     levels = 0;
+    lines = 0;
     out.resize(in.size());
     for (size_t q = 0; q < order.size(); ++q) {
         const size_t i = order[q];
@@ -185,9 +206,16 @@ uint64_t assemble_queue(const std::vector<QueuedPass>& in, int order_mode, int f
         bp.kcase = bc ? (rk_u32)bc->id : 0u;   // build_chain checked the case exists in the blue flavour
         bp.src = qp.src;
         bp.dst = qp.dst;
-        bp.wait = cp.first ? -1 : pos[i - 1];   // the item's previous pass
-        bp.wait_tiles = cp.first ? 0u : (rk_u32)in[i - 1].a.n_tiles;
-        bp.signal = cp.last ? 0u : 1u;
+        // the item's previous pass (earlier in the queue: it has a lower level)
+        // is the only one it waits for, and it is the only pass that waits for
+        // that one
+        bp.wait = cp.first ? -1 : pos[i - 1];
+        bp.awaited = cp.last ? 0u : 1u;
+        if (!(cp.first && cp.last)) {
+            bp.groups = tile_groups(qp.a.n_tiles);
+            bp.line = (rk_u32)lines;
+            lines += 1 + bp.groups;
+        }
         bp.item_first = cp.first ? 1u : 0u;
         bp.item_last = cp.last ? 1u : 0u;
         bp.fwd_scale = cp.fwd_scale;
@@ -196,6 +224,11 @@ uint64_t assemble_queue(const std::vector<QueuedPass>& in, int order_mode, int f
         total += qp.a.n_tiles;
         levels = std::max(levels, qp.level + 1);
     }
+    for (rk_batch_pass& bp : out)
+        if (bp.wait >= 0) {
+            bp.wait_line = out[bp.wait].line;
+            bp.wait_groups = out[bp.wait].groups;
+        }
     tile_pass.resize(total);
     for (size_t q = 0; q < out.size(); ++q)
         for (rk_u64 t = 0; t < out[q].a.n_tiles; ++t) tile_pass[out[q].first + t] = (rk_u32)q;
@@ -224,7 +257,7 @@ struct rk_batch_plan {
     std::vector<rk_batch_pass> passes;   // queue order
     std::vector<rk_u32> tile_pass;
     std::vector<rk::CopyArgs> copies;    // length-1 items
-    void* dev = nullptr;                 // tables | passes | tile_pass | done | seen | state
+    void* dev = nullptr;                 // tables | passes | tile_pass | counters | state
     rk_batch_work work;
     int flavour = rk::BATCH_LIGHT;
     int64_t n_items = 0;
@@ -349,23 +382,26 @@ static int create_queue(const rk_batch_item* items, int n_items, rk_batch_plan* 
     plan->work.n_passes = (rk_u32)queued.size();
     plan->work.total = (rk_u32)total;
     plan->work.spin_limit = kSpinLimit;
+    plan->work.poll_sleep_max = rk::batch_poll_sleep_max();
+    size_t lines = 0;
     if (plan->dry) {
-        rk::assemble_queue(queued, plan->opt.order, plan->flavour, plan->passes, plan->tile_pass, plan->levels);
+        rk::assemble_queue(queued, plan->opt.order, plan->flavour, plan->passes, plan->tile_pass, plan->levels, lines);
         return RK_OK;
     }
 
-    // one device allocation: tables | passes | tile_pass | done | seen | state
+    // one device allocation: tables | passes | tile_pass | counters | state
     auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
     const size_t off_pass = align(tb.bytes.size());
     const size_t off_tp = off_pass + align(queued.size() * sizeof(rk_batch_pass));
-    const size_t off_done = off_tp + align((size_t)total * sizeof(rk_u32));
-    const size_t off_seen = off_done + align(queued.size() * sizeof(rk_u32));
-    const size_t off_state = off_seen + align(queued.size() * sizeof(rk_u32));
+    const size_t off_cnt = off_tp + align((size_t)total * sizeof(rk_u32));
+    for (auto& qp : queued) rk::resolve_tables(qp.a, qp.cp->pp, nullptr);
+    rk::assemble_queue(queued, plan->opt.order, plan->flavour, plan->passes, plan->tile_pass, plan->levels, lines);
+    const size_t off_state = off_cnt + align(lines * RK_BATCH_LINE * sizeof(rk_u32));
     const size_t bytes = off_state + 256;
     RK_HIP(hipMalloc(&plan->dev, bytes));
     char* base = (char*)plan->dev;
     for (auto& qp : queued) rk::resolve_tables(qp.a, qp.cp->pp, base);
-    rk::assemble_queue(queued, plan->opt.order, plan->flavour, plan->passes, plan->tile_pass, plan->levels);
+    rk::assemble_queue(queued, plan->opt.order, plan->flavour, plan->passes, plan->tile_pass, plan->levels, lines);
     if (!tb.bytes.empty()) RK_HIP(hipMemcpy(base, tb.bytes.data(), tb.bytes.size(), hipMemcpyHostToDevice));
     if (!plan->passes.empty())
         RK_HIP(hipMemcpy(base + off_pass, plan->passes.data(), plan->passes.size() * sizeof(rk_batch_pass),
@@ -373,14 +409,13 @@ static int create_queue(const rk_batch_item* items, int n_items, rk_batch_plan* 
     if (!plan->tile_pass.empty())
         RK_HIP(hipMemcpy(base + off_tp, plan->tile_pass.data(), plan->tile_pass.size() * sizeof(rk_u32),
                          hipMemcpyHostToDevice));
-    RK_HIP(hipMemset(base + off_done, 0, bytes - off_done));
+    RK_HIP(hipMemset(base + off_cnt, 0, bytes - off_cnt));
     // the plan executes on callers' streams, which do not wait for the null
     // stream: the uploads above must have landed before this returns
     RK_HIP(hipDeviceSynchronize());
     plan->work.passes = (const rk_batch_pass*)(base + off_pass);
     plan->work.tile_pass = (const rk_u32*)(base + off_tp);
-    plan->work.done = (rk_u32*)(base + off_done);
-    plan->work.seen = (rk_u32*)(base + off_seen);
+    plan->work.counters = (rk_u32*)(base + off_cnt);
     plan->work.state = (rk_u32*)(base + off_state);
     return RK_OK;
 }

@@ -62,7 +62,12 @@ struct QueuedPass {
 //   1  level by level, items in order;
 //   2  level by level, largest tiles first.
 uint64_t assemble_queue(const std::vector<QueuedPass>& in, int order_mode, int flavour,
-                        std::vector<rk_batch_pass>& passes, std::vector<rk_u32>& tile_pass, int& levels);
+                        std::vector<rk_batch_pass>& passes, std::vector<rk_u32>& tile_pass, int& levels,
+                        size_t& counter_lines);
+
+// Longest back-off between two polls of a waiting tile, in units of 128
+// clocks (RK_BATCH_POLL_MAX overrides; default 16, ~0.85 us).
+rk_u32 batch_poll_sleep_max();
 
 // Resolves table offsets of a pass (from build_chain's TableBuilder) against
 // the device copy of the table block.

@@ -5,7 +5,8 @@
 // shape (reikna/fft/fft.py:723-756).  The queue protocol is described in
 // fft_batch.h; the grid has one work-group per queue position and no atomic
 // is shared by the whole grid (same-address device atomics serialise at
-// ~11 ns each on MI355X: 94 us for one per work-group at 8192 work-groups).
+// ~11 ns each on MI355X: 94 us for one per work-group at 8192 work-groups),
+// and the tiles of one pass count themselves on up to 16 counters.
 // All data accesses of the kernel are device-coherent (CoherentIO), so a
 // pass hands its output to the next pass of the same transform through a
 // tile counter alone: agent-scope fences would write back or invalidate a
@@ -64,9 +65,13 @@ __device__ __forceinline__ void run_case(rk_u32 kc, const rk_pass_args& a, float
     }
 }
 
+__device__ __forceinline__ void clear_counters(rk_u32* c, rk_u32 groups) {
+    for (rk_u32 g = 0; g <= groups; ++g) __hip_atomic_store(c + g * RK_BATCH_LINE, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 }  // namespace
 
-extern "C" __global__ void __launch_bounds__(kThreads) RK_BATCH_KERNEL(rk_batch_work w) {
+extern "C" __global__ void __launch_bounds__(kThreads, RK_BATCH_MIN_WAVES) RK_BATCH_KERNEL(rk_batch_work w) {
     __shared__ float2 lds[(kLdsBytes + 7) / 8];
     const int tid = threadIdx.x;
     // queue position = work-group index: work-groups are dispatched in index
@@ -78,22 +83,20 @@ extern "C" __global__ void __launch_bounds__(kThreads) RK_BATCH_KERNEL(rk_batch_
     const rk_i32 wait = bp->wait;
     if (wait >= 0) {
         if (tid == 0) {
-            const rk_u32 need = bp->wait_tiles;
-            rk_u32 polls = 0;
-            while (__hip_atomic_load(&w.done[wait], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+            // poll the awaited pass's finished-group count, backing off from
+            // ~50 ns to poll_sleep_max x ~50 ns between polls (a queue holds
+            // many waiting work-groups; fewer polls leave the memory channel
+            // of the counter to the atomics that move it)
+            const rk_u32* top = w.counters + (size_t)bp->wait_line * RK_BATCH_LINE;
+            const rk_u32 need = bp->wait_groups;
+            rk_u32 polls = 0, nap = 1;
+            while (__hip_atomic_load(top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
                 if (++polls > w.spin_limit) {
                     __hip_atomic_store(&w.state[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     break;
                 }
-                __builtin_amdgcn_s_sleep(4);
-            }
-            // the last tile of this pass to get past the wait clears the
-            // producer's counters for the next launch (every tile of this
-            // pass has seen the count by then)
-            const rk_u32 seen = __hip_atomic_fetch_add(&w.seen[wait], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (seen + 1 == (rk_u32)bp->a.n_tiles) {
-                __hip_atomic_store(&w.done[wait], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(&w.seen[wait], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (rk_u32 k = 0; k < nap; ++k) __builtin_amdgcn_s_sleep(2);
+                if (nap < w.poll_sleep_max) nap <<= 1;
             }
         }
         __syncthreads();
@@ -107,12 +110,27 @@ extern "C" __global__ void __launch_bounds__(kThreads) RK_BATCH_KERNEL(rk_batch_
     a.out_sign = (inv && bp->item_last) ? -1.0f : 1.0f;
     a.out_scale = inv ? bp->inv_scale : bp->fwd_scale;
     run_case(bp->kcase, a, lds, (rk_u64)(item - bp->first), tid);
-    if (bp->signal) {
+    const rk_u32 groups = bp->groups;
+    if (groups) {
         // every store of the tile has completed (they are device-coherent)
         // before the tile is counted
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        if (tid == 0) __hip_atomic_fetch_add(&w.done[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (tid == 0) {
+            rk_u32* own = w.counters + (size_t)bp->line * RK_BATCH_LINE;
+            const rk_u32 t = item - (rk_u32)bp->first;
+            const rk_u32 n = (rk_u32)bp->a.n_tiles;
+            const rk_u32 g = t % groups;
+            const rk_u32 size = n / groups + (g < n % groups ? 1u : 0u);
+            if (__hip_atomic_fetch_add(own + (1 + g) * RK_BATCH_LINE, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 == size &&
+                __hip_atomic_fetch_add(own, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 == groups) {
+                // every tile of this pass is done: nothing reads the awaited
+                // pass's counters any more, nor this pass's when no later pass
+                // waits for it
+                if (wait >= 0) clear_counters(w.counters + (size_t)bp->wait_line * RK_BATCH_LINE, bp->wait_groups);
+                if (!bp->awaited) clear_counters(own, groups);
+            }
+        }
     }
 }
 

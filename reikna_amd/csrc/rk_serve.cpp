@@ -126,8 +126,8 @@ struct Request {
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    void* dev = nullptr;             // passes | tile_pass | done | seen | state
-    size_t cap_passes = 0, cap_tiles = 0;
+    void* dev = nullptr;             // passes | tile_pass | counters | state
+    size_t cap_passes = 0, cap_tiles = 0, cap_lines = 0;
     std::vector<rk_batch_pass> passes;
     std::vector<rk_u32> tile_pass;
     std::vector<Request> reqs;       // in flight
@@ -324,16 +324,19 @@ static int launch_batched(rk_server* s, Slot& sl) {
     int levels = 0, flavour = rk::BATCH_LIGHT;
     for (const Request& q : sl.reqs)
         if (q.t->chain.blue) flavour = rk::BATCH_BLUE;
-    const uint64_t total = rk::assemble_queue(queued, 0, flavour, sl.passes, sl.tile_pass, levels);
+    size_t lines = 0;
+    const uint64_t total = rk::assemble_queue(queued, 0, flavour, sl.passes, sl.tile_pass, levels, lines);
     if (total == 0) return RK_OK;
-    // device buffers of the slot (grown as needed; the kernel leaves done /
-    // state zero at its end)
-    if (sl.passes.size() > sl.cap_passes || total > sl.cap_tiles) {
+    // device buffers of the slot (grown as needed; the kernel leaves its
+    // counters and state zero at its end)
+    const size_t line_bytes = RK_BATCH_LINE * sizeof(rk_u32);
+    if (sl.passes.size() > sl.cap_passes || total > sl.cap_tiles || lines > sl.cap_lines) {
         RK_HIP(hipStreamSynchronize(sl.stream));
         if (sl.dev) RK_HIP(hipFree(sl.dev));
         sl.cap_passes = std::max<size_t>(sl.passes.size() * 2, 1024);
         sl.cap_tiles = std::max<size_t>(total * 2, 16384);
-        const size_t bytes = sl.cap_passes * sizeof(rk_batch_pass) + sl.cap_tiles * 4 + 2 * sl.cap_passes * 4 + 256;
+        sl.cap_lines = std::max<size_t>(lines * 2, 1024);
+        const size_t bytes = sl.cap_passes * sizeof(rk_batch_pass) + sl.cap_tiles * 4 + sl.cap_lines * line_bytes + 256;
         RK_HIP(hipMalloc(&sl.dev, bytes));
         // on the slot's stream: ordered before the descriptor uploads below
         // (the slot's stream does not wait for the null stream)
@@ -345,12 +348,12 @@ static int launch_batched(rk_server* s, Slot& sl) {
     w.passes = (const rk_batch_pass*)base;
     w.tile_pass = (const rk_u32*)(base + sl.cap_passes * sizeof(rk_batch_pass));
     char* counters = base + sl.cap_passes * sizeof(rk_batch_pass) + sl.cap_tiles * 4;
-    w.done = (rk_u32*)counters;
-    w.seen = (rk_u32*)(counters + sl.cap_passes * 4);
-    w.state = (rk_u32*)(counters + 2 * sl.cap_passes * 4);
+    w.counters = (rk_u32*)counters;
+    w.state = (rk_u32*)(counters + sl.cap_lines * line_bytes);
     w.n_passes = (rk_u32)sl.passes.size();
     w.total = (rk_u32)total;
     w.spin_limit = kSpinLimit;
+    w.poll_sleep_max = rk::batch_poll_sleep_max();
     RK_HIP(hipMemcpyAsync((void*)w.passes, sl.passes.data(), sl.passes.size() * sizeof(rk_batch_pass),
                           hipMemcpyHostToDevice, sl.stream));
     RK_HIP(hipMemcpyAsync((void*)w.tile_pass, sl.tile_pass.data(), sl.tile_pass.size() * 4, hipMemcpyHostToDevice,
