@@ -75,10 +75,24 @@ def lds_pitch(length: int) -> int:
     return s + 1 if s % 2 == 0 else s
 
 
+def tile_pitch(length: int, c: int) -> int:
+    """Row pitch of a tile (fft_device.hpp lds_pitch_for)."""
+    split = c * lds_pitch(length) * 8 > 80 * 1024
+    pow2 = c > 0 and c & (c - 1) == 0
+    if not pow2 or c < 2 or threads_per_instance(length) < 16 or c > (32 if split else 16):
+        return lds_pitch(length)
+    m = 64 if split else 32
+    p = (length - 1) + ((length - 1) >> 4) + 1
+    while p % m != m // c:
+        p += 1
+    return p
+
+
 def tile_lds_bytes(length: int, c: int) -> int:
-    """LDS of a tile: the exchanges go through halves (floats) above 80 KB."""
-    full = c * lds_pitch(length) * 8
-    return full // 2 if full > 80 * 1024 else full
+    """LDS of a tile: the exchanges go through halves (floats) when the tile
+    needs more than 80 KB at the base pitch."""
+    split = c * lds_pitch(length) * 8 > 80 * 1024
+    return c * tile_pitch(length, c) * (4 if split else 8)
 
 
 def threads_per_instance(length: int) -> int:

@@ -30,15 +30,23 @@ using rk::fail;
 
 namespace rk {
 
-// Longest single-pass DFT (RK_SINGLE_MAX overrides, at most 16384).
-static int64_t single_max() {
+// Longest single-pass DFT of a smooth length, and longest single-kernel
+// Bluestein convolution (RK_SINGLE_MAX overrides both, at most 16384).  Above
+// 4096 points a single-pass tile is one instance of 512+ threads with 64+ KB
+// of LDS, one or two work-groups per CU, and streams slower than two passes
+// of short DFTs: 4096 x 12288 took 413 us in one pass and 345 us in two
+// (96 x 128).  A single-kernel Bluestein convolution reads and writes its
+// data once instead of running two chains through a workspace of twice the
+// length, so it keeps the longer limit.
+static int64_t env_single_max() {
     static const int64_t v = [] {
         const char* e = getenv("RK_SINGLE_MAX");
-        const int64_t x = e && *e ? (int64_t)atoll(e) : (int64_t)16384;
-        return std::min<int64_t>(std::max<int64_t>(x, 2), 16384);
+        return e && *e ? std::min<int64_t>(std::max<int64_t>(atoll(e), 2), 16384) : (int64_t)0;
     }();
     return v;
 }
+static int64_t single_max() { return env_single_max() ? env_single_max() : 4096; }
+static int64_t blue_single_max() { return env_single_max() ? env_single_max() : 16384; }
 #define kSingleMax (single_max())
 
 // Longest DFT of one pass in a multi-pass chain.  Column-style passes need
@@ -88,7 +96,7 @@ ChainLimits default_limits() {
     l.single_max = single_max();
     l.factor_max = factor_max();
     l.pow2_bluestein = false;
-    l.blue_single_max = l.single_max;
+    l.blue_single_max = blue_single_max();
     l.length_ok = nullptr;
     l.strided_single_max = 0;
     return l;
@@ -119,9 +127,10 @@ int threads_per_instance(int L) {
     return L / rl.r[0];
 }
 
-int lds_pitch_host(int L) {
-    int s = rkfft::lds_pad(L - 1) + 1;
-    return (s % 2 == 0) ? s + 1 : s;
+int lds_pitch_host(int L) { return rkfft::lds_pitch_base(L); }
+
+int tile_lds_bytes_host(int L, int C) {
+    return C * rkfft::lds_pitch_for(L, C, threads_per_instance(L)) * (rkfft::lds_split(L, C) ? 4 : 8);
 }
 
 int tile_c_row(int L) {
@@ -475,6 +484,39 @@ int describe_pass(const AxisPass& ap, const Layout& ls, const Layout& ld, int ax
 
 static int plan_tile_width(int L, bool colmap) { return colmap ? tile_c_col(L) : tile_c_row(L); }
 
+// XCDs of the current device (the dispatcher hands work-group b to XCD
+// b % count); 8 on MI355X when there is no device to ask.
+static rk_u32 xcd_count() {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&n, hipDeviceAttributeNumberOfXccs, dev) == hipSuccess &&
+        n > 0)
+        return (rk_u32)n;
+    (void)hipGetLastError();
+    return 8;
+}
+
+// Column tiles narrower than 16 instances share 128-byte lines with their
+// neighbours: each group of 32 / C consecutive tiles is sent to one XCD, so
+// the neighbours meet in that XCD's L2 instead of each fetching whole lines
+// from HBM (RK_XCD_GROUP overrides the group size; 1 turns it off).  Measured
+// on the two 1024-point passes of a 64 x 2^20 transform with C = 4: 941 us
+// without groups, 683 us with groups of 4, 632 us with groups of 8.
+static void set_xcd_group(PassPlan& pp) {
+    static const int forced = [] {
+        const char* e = getenv("RK_XCD_GROUP");
+        return e && *e ? atoi(e) : 0;
+    }();
+    rk_u32 g = 1;
+    if (forced > 0)
+        g = (rk_u32)forced;
+    else if (pp.args.cmap && pp.C < 16)
+        g = (rk_u32)(32 / pp.C);
+    if (g > 1) {
+        pp.args.xcd_group = g;
+        pp.args.n_xcd = xcd_count();
+    }
+}
+
 static int find_kernel(PassPlan& pp, uint32_t flags) {
     for (const KernelEntry& e : catalog())
         if (e.L == pp.L && e.C == pp.C && e.variant == pp.variant) {
@@ -488,8 +530,7 @@ static int find_kernel(PassPlan& pp, uint32_t flags) {
                     pp.L, pp.C, pp.variant);
     if (flags & RK_FFT_DRY_RUN) {   // describe the kernel the run-time compiler would build
         pp.threads = pp.C * threads_per_instance(pp.L);
-        const int full = pp.C * lds_pitch_host(pp.L) * 8;
-        pp.lds_bytes = full > 80 * 1024 ? full / 2 : full;
+        pp.lds_bytes = tile_lds_bytes_host(pp.L, pp.C);
         return RK_OK;
     }
     int rc = jit_pass_kernel(pp.L, pp.C, pp.variant, &pp.jit, &pp.threads, &pp.lds_bytes);
@@ -687,6 +728,7 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
     PassHook hook = [&](PassPlan& pp, const AxisPass& ap, int64_t n) -> int {
         int rc = find_kernel(pp, d->flags);
         if (rc != RK_OK) return rc;
+        set_xcd_group(pp);
         const uint64_t max_grid = (uint64_t(1) << 31) / (uint64_t)pp.threads;
         pp.grid = std::min<uint64_t>(pp.args.n_tiles, max_grid);
         return build_tables(tb, pp, ap, n);

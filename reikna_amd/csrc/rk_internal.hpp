@@ -54,7 +54,8 @@ const std::vector<KernelEntry>& catalog();
 // Work-group tile widths the planner asks for (mirrors the catalog generator).
 int tile_c_row(int L);
 int tile_c_col(int L);
-int lds_pitch_host(int L);
+int lds_pitch_host(int L);            // base pitch (tile width choices)
+int tile_lds_bytes_host(int L, int C);   // LDS of a tile (fft_device.hpp tile_lds_bytes)
 int threads_per_instance(int L);
 bool is_smooth(int64_t n);
 

@@ -66,9 +66,14 @@ def test_plan_bluestein_length(vdev, n, expect):
 
 
 def test_plan_smooth_non_pow2_is_not_bluestein(vdev):
-    for n in (1000, 3 * 2**12, 5 * 2**11, 7 * 2**9):
+    for n in (1000, 7 * 2**9):
         ps = passes((4, n), (1,), dev=vdev)
         assert len(ps) == 1 and ps[0].kind == 0 and ps[0].L == n
+    # above 4096 points: two Stockham passes, no Bluestein padding
+    for n in (3 * 2**12, 5 * 2**11):
+        ps = passes((4, n), (1,), dev=vdev)
+        assert len(ps) == 2 and all(p.kind == 0 and p.axis_len == n for p in ps)
+        assert ps[0].L * ps[1].L == n
 
 
 def test_plan_flags(vdev):
