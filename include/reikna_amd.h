@@ -236,10 +236,19 @@ typedef struct {
     int64_t out_offset;  /* element offset of the first transform in `out` */
     int64_t in_dist;     /* elements between consecutive transforms (>= N) */
     int64_t out_dist;
-    uint32_t flags;      /* RK_BATCH_SCALE: multiply result by 1/N */
+    uint32_t flags;      /* RK_BATCH_SCALE: multiply result by 1/N;
+                            RK_BATCH_R2C: real-to-complex item (below) */
 } rk_batch_item;
 
 #define RK_BATCH_SCALE 0x1u
+/* Real-to-complex item (numpy.fft.rfft; the r2c packing of
+   examples/demo_specialized_fft.py:156-331 run inside the batch kernel):
+   `length` N is even and >= 4; transform j reads N float32 values starting
+   at complex64 element in_offset + j * in_dist of `in` (read as N/2 complex
+   values, so in_dist >= N/2) and writes the N/2 + 1 non-negative frequencies
+   at out_offset + j * out_dist (out_dist >= N/2 + 1).  Forward only, queue
+   mode only. */
+#define RK_BATCH_R2C 0x2u
 
 /* Execution modes: one launch of the work-queue kernel for the whole batch
    (default), or one single-shape plan per item (the per-request path). */

@@ -81,8 +81,6 @@ class ServerStats(ctypes.Structure):
 RK_SERVER_BATCHED = 0
 RK_SERVER_PLAIN = 1
 
-RK_BATCH_SCALE = 0x1
-
 
 class RkError(RuntimeError):
     """A failed call into libreikna_amd (HIP error, missing kernel, ...)."""
@@ -190,6 +188,7 @@ class BatchInfo(ctypes.Structure):
 
 
 RK_BATCH_SCALE = 0x1
+RK_BATCH_R2C = 0x2
 RK_BATCH_MODE_QUEUE = 0
 RK_BATCH_MODE_GROUPED = 1
 RK_BATCH_DRY_RUN = 0x1

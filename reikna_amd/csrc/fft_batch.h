@@ -35,6 +35,14 @@
 #define RK_BATCH_BUF_WS 2u
 #define RK_BATCH_BUF_ABS 3u   // the pointer already in the pass arguments
 
+// Tile case of the real-to-complex combine pass (not an FFT tile): tile t
+// of the pass computes outputs k = (t % a.dim_I) * RK_BATCH_R2C_SPAN + ... of
+// transform t / a.dim_I, X[k] = Z[k] A[k] + conj(Z[N/2 - k]) B[k] from the
+// half-length spectrum Z (a.in, a.N = N/2 points per transform, a.in_so
+// apart) with A = a.chirp, B = a.bspec, and X[N/2] = Re Z[0] - Im Z[0].
+#define RK_BATCH_CASE_R2C 0xffffu
+#define RK_BATCH_R2C_SPAN (4 * RK_BATCH_THREADS)
+
 #define RK_BATCH_LINE 32u         // u32 per counter line (128 bytes)
 #define RK_BATCH_GROUPS_MAX 16u   // tile groups of one pass
 

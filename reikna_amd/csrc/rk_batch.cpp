@@ -159,6 +159,60 @@ static rk_u32 tile_groups(uint64_t n) {
     return g;
 }
 
+// Appends the real-to-complex combine pass to the chain of the half-length
+// complex FFT of an RK_BATCH_R2C item (N real points, `count` transforms,
+// output at out_offset + j * out_dist) and its A / B tables (FP64, rounded
+// once).  The chain's own output (BUF_OUT) is the half-length spectrum Z,
+// `count` transforms of N/2 points back to back; the combine pass reads it
+// from offset 0 of that region.
+int append_r2c_combine(int64_t n, int64_t count, int64_t out_offset, int64_t out_dist, bool scale, TableBuilder& tb,
+                       Chain& chain) {
+    const int64_t half = n / 2;
+    std::vector<float2> ta((size_t)half), tbv((size_t)half);
+    for (int64_t k = 0; k < half; ++k) {
+        // W = exp(-2 pi i k / N), k < N/2: A = (1 - iW)/2, B = (1 + iW)/2
+        const double ang = -2.0 * M_PI * (double)k / (double)n;
+        const double c = std::cos(ang), sn = std::sin(ang);
+        ta[(size_t)k] = make_float2((float)(0.5 * (1.0 + sn)), (float)(-0.5 * c));
+        tbv[(size_t)k] = make_float2((float)(0.5 * (1.0 - sn)), (float)(0.5 * c));
+    }
+    ChainPass cp;
+    memset(&cp.pp.args, 0, sizeof(cp.pp.args));
+    rk_pass_args& a = cp.pp.args;
+    const rk_u64 chunks = (rk_u64)((half + 1 + RK_BATCH_R2C_SPAN - 1) / RK_BATCH_R2C_SPAN);
+    a.N = (rk_u64)half;
+    a.n_inst = (rk_u64)count;
+    a.dim_I = chunks;
+    a.n_tiles = (rk_u64)count * chunks;
+    a.in_off = 0;
+    a.in_so = half;
+    a.in_sa = 1;
+    a.out_off = out_offset;
+    a.out_so = out_dist;
+    a.out_sa = 1;
+    a.in_sign = a.out_sign = a.out_scale = 1.0f;
+    a.n_valid_in = a.n_valid_out = (rk_u64)half;
+    cp.pp.src = BUF_OUT;   // the chain's output region (Z)
+    cp.pp.dst = BUF_OUT;
+    cp.pp.L = 1;
+    cp.pp.C = RK_BATCH_R2C_SPAN;
+    cp.pp.variant = VAR_R2C;
+    cp.pp.threads = RK_BATCH_THREADS;
+    const std::string key = "r2c:" + std::to_string(n);
+    cp.pp.chirp_off = tb.add(key + ":A", ta.data(), ta.size() * sizeof(float2));
+    cp.pp.bspec_off = tb.add(key + ":B", tbv.data(), tbv.size() * sizeof(float2));
+    cp.first = false;
+    cp.last = true;
+    cp.fwd_scale = scale ? (float)(1.0 / (double)n) : 1.0f;
+    cp.inv_scale = 1.0f;
+    if (chain.passes.empty()) return fail(RK_ERR_INVALID, "batch: r2c item without a complex chain");
+    ChainPass& prev = chain.passes.back();
+    prev.last = false;
+    prev.fwd_scale = prev.inv_scale = 1.0f;
+    chain.passes.push_back(cp);
+    return RK_OK;
+}
+
 uint64_t assemble_queue(const std::vector<QueuedPass>& in, int order_mode, int flavour,
                         std::vector<rk_batch_pass>& out, std::vector<rk_u32>& tile_pass, int& levels,
                         size_t& lines) {
@@ -202,8 +256,12 @@ uint64_t assemble_queue(const std::vector<QueuedPass>& in, int order_mode, int f
         memset(&bp, 0, sizeof(bp));
         bp.a = qp.a;
         bp.first = total;
-        const BatchCase* bc = find_case(cp.pp.L, cp.pp.variant, flavour);
-        bp.kcase = bc ? (rk_u32)bc->id : 0u;   // build_chain checked the case exists in the blue flavour
+        if (cp.pp.variant == VAR_R2C) {
+            bp.kcase = RK_BATCH_CASE_R2C;
+        } else {
+            const BatchCase* bc = find_case(cp.pp.L, cp.pp.variant, flavour);
+            bp.kcase = bc ? (rk_u32)bc->id : 0u;   // build_chain checked the case exists in the blue flavour
+        }
         bp.src = qp.src;
         bp.dst = qp.dst;
         // the item's previous pass (earlier in the queue: it has a lower level)
@@ -265,6 +323,7 @@ struct rk_batch_plan {
     size_t ws_bytes = 0;
     int launches = 0;
     bool dry = false;
+    bool r2c = false;                    // has real-to-complex items (forward only)
 };
 
 extern "C" int rk_batch_plan_destroy(rk_batch_plan* plan) {
@@ -279,6 +338,8 @@ static int create_grouped(const rk_batch_item* items, int n_items, rk_batch_plan
     for (int k = 0; k < n_items; ++k) {
         const rk_batch_item& it = items[k];
         if (it.count == 0) continue;
+        if (it.flags & RK_BATCH_R2C)
+            return fail(RK_ERR_UNSUPPORTED, "batch item %d: real-to-complex items run in queue mode only", k);
         rk_fft_desc d;
         memset(&d, 0, sizeof(d));
         d.dtype = RK_C64;
@@ -333,21 +394,32 @@ static int create_queue(const rk_batch_item* items, int n_items, rk_batch_plan* 
             plan->copies.push_back(c);
             continue;
         }
+        // a real-to-complex item is the complex FFT of its N/2 pairs into a
+        // workspace region Z, then the combine pass from Z to `out`
+        const bool r2c = (it.flags & RK_BATCH_R2C) != 0;
+        const int64_t n = r2c ? it.length / 2 : it.length;
         rk::ItemShape sh;
         sh.nd = 1;
-        sh.n[0] = it.length;
+        sh.n[0] = n;
         sh.count = it.count;
         sh.in_offset = it.in_offset;
-        sh.out_offset = it.out_offset;
-        sh.in_dist = it.count > 1 ? it.in_dist : it.length;
-        sh.out_dist = it.count > 1 ? it.out_dist : it.length;
-        sh.scale = (it.flags & RK_BATCH_SCALE) != 0;
+        sh.in_dist = it.count > 1 ? it.in_dist : n;
+        sh.out_offset = r2c ? 0 : it.out_offset;
+        sh.out_dist = r2c ? n : it.count > 1 ? it.out_dist : n;
+        sh.scale = !r2c && (it.flags & RK_BATCH_SCALE) != 0;
         plan->chains.emplace_back();
         rk::Chain& ch = plan->chains.back();
         int rc = rk::build_chain(sh, lim, tb, ch);
         if (rc != RK_OK) return rc;
-        // this item's workspace region: [ws_elems, ws_elems + ws_total)
+        if (r2c) {
+            rc = rk::append_r2c_combine(it.length, it.count, it.out_offset, it.count > 1 ? it.out_dist : n + 1,
+                                        (it.flags & RK_BATCH_SCALE) != 0, tb, ch);
+            if (rc != RK_OK) return rc;
+            plan->r2c = true;
+        }
+        // this item's workspace regions: [ws_elems, + ws_total), then Z
         const int64_t base0 = (int64_t)ws_elems, base1 = (int64_t)(ws_elems + ch.ws_elems);
+        const int64_t base_z = (int64_t)(ws_elems + ((ch.ws_total() + 31) & ~size_t(31)));
         for (size_t i = 0; i < ch.passes.size(); ++i) {
             const rk::ChainPass& cp = ch.passes[i];
             rk::QueuedPass qp;
@@ -358,15 +430,22 @@ static int create_queue(const rk_batch_item* items, int n_items, rk_batch_plan* 
             qp.inverse = 0;
             auto map = [&](int buf, rk_i64& off) -> rk_u32 {
                 if (buf == rk::BUF_IN) return RK_BATCH_BUF_IN;
-                if (buf == rk::BUF_OUT) return RK_BATCH_BUF_OUT;
-                off += buf == rk::BUF_WS0 ? base0 : base1;
+                if (buf == rk::BUF_OUT && !r2c) return RK_BATCH_BUF_OUT;
+                off += buf == rk::BUF_WS0 ? base0 : buf == rk::BUF_WS1 ? base1 : base_z;
                 return RK_BATCH_BUF_WS;
             };
-            qp.src = map(cp.pp.src, qp.a.in_off);
-            qp.dst = map(cp.pp.dst, qp.a.out_off);
+            if (cp.pp.variant == rk::VAR_R2C) {
+                qp.a.in_off += base_z;
+                qp.src = RK_BATCH_BUF_WS;
+                qp.dst = RK_BATCH_BUF_OUT;
+            } else {
+                qp.src = map(cp.pp.src, qp.a.in_off);
+                qp.dst = map(cp.pp.dst, qp.a.out_off);
+            }
             queued.push_back(qp);
         }
         ws_elems += (ch.ws_total() + 31) & ~size_t(31);   // 256-byte aligned regions
+        if (r2c) ws_elems += ((size_t)(n * it.count) + 31) & ~size_t(31);
     }
 
     uint64_t total = 0;
@@ -427,9 +506,16 @@ extern "C" int rk_batch_plan_create_ex(const rk_batch_item* items, int n_items, 
     for (int k = 0; k < n_items; ++k) {
         const rk_batch_item& it = items[k];
         if (it.length < 1 || it.count < 0) return fail(RK_ERR_INVALID, "batch item %d: bad length/count", k);
-        if (it.count > 1 && (it.in_dist < it.length || it.out_dist < it.length))
+        if (!(it.flags & RK_BATCH_R2C) && it.count > 1 && (it.in_dist < it.length || it.out_dist < it.length))
             return fail(RK_ERR_INVALID, "batch item %d: distance smaller than the length", k);
         if (it.in_offset < 0 || it.out_offset < 0) return fail(RK_ERR_INVALID, "batch item %d: negative offset", k);
+        if (it.flags & ~(RK_BATCH_SCALE | RK_BATCH_R2C)) return fail(RK_ERR_INVALID, "batch item %d: unknown flags", k);
+        if (it.flags & RK_BATCH_R2C) {
+            if (it.length < 4 || it.length % 2)
+                return fail(RK_ERR_INVALID, "batch item %d: a real-to-complex length must be even and >= 4", k);
+            if (it.count > 1 && (it.in_dist < it.length / 2 || it.out_dist < it.length / 2 + 1))
+                return fail(RK_ERR_INVALID, "batch item %d: distances smaller than N/2 (in) or N/2 + 1 (out)", k);
+        }
     }
     std::unique_ptr<rk_batch_plan, int (*)(rk_batch_plan*)> plan(new rk_batch_plan(), rk_batch_plan_destroy);
     if (options) {
@@ -466,6 +552,8 @@ extern "C" int rk_batch_execute(const rk_batch_plan* plan, void* out, const void
     if (!out || !in) return fail(RK_ERR_INVALID, "rk_batch_execute: null in/out pointer");
     if (plan->ws_bytes && !workspace)
         return fail(RK_ERR_INVALID, "rk_batch_execute: plan needs %zu bytes of workspace", plan->ws_bytes);
+    if (inverse && plan->r2c)
+        return fail(RK_ERR_UNSUPPORTED, "rk_batch_execute: real-to-complex items are forward only");
     hipStream_t s = (hipStream_t)stream;
     if (plan->opt.mode == RK_BATCH_MODE_GROUPED) {
         for (auto* p : plan->plans) {

@@ -44,6 +44,12 @@ ChainLimits batch_limits(const rk_batch_options* options);
 // whose passes have no tile case) and appends its tables to tb.
 int build_chain(const ItemShape& item, const ChainLimits& lim, TableBuilder& tb, Chain& chain);
 
+// Appends the combine pass of a real-to-complex item of N real points to
+// the chain of its N/2-point complex FFT (whose BUF_OUT is then the
+// half-length spectrum, N/2 points per transform back to back).
+int append_r2c_combine(int64_t n, int64_t count, int64_t out_offset, int64_t out_dist, bool scale, TableBuilder& tb,
+                       Chain& chain);
+
 // One pass placed in a queue: its arguments with buffer offsets applied.
 struct QueuedPass {
     const ChainPass* cp;

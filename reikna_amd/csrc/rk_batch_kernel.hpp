@@ -65,6 +65,44 @@ __device__ __forceinline__ void run_case(rk_u32 kc, const rk_pass_args& a, float
     }
 }
 
+// Real-to-complex combine tile (RK_BATCH_CASE_R2C, fft_batch.h): Z is the
+// N/2-point spectrum of the real input read as complex pairs x[n] = a[2n] +
+// i a[2n+1]; X[k] = Z[k] A[k] + conj(Z[(N/2 - k) mod N/2]) B[k] for k < N/2
+// and X[N/2] = Re Z[0] - Im Z[0] (Rabiner's split of a real FFT, the one
+// examples/demo_specialized_fft.py:156-331 fuses; A, B from FP64).  Every
+// load is issued before the first store.
+__device__ __forceinline__ void r2c_combine_tile(const rk_pass_args& a, rk_u64 tile, int tid) {
+    const rkfft::CoherentIO io;
+    const rk_u64 chunks = a.dim_I;
+    const rk_u64 inst = tile / chunks;
+    const rk_u64 k0 = (tile - inst * chunks) * RK_BATCH_R2C_SPAN;
+    const rk_u64 half = a.N;
+    const rk_i64 zb = a.in_off + (rk_i64)inst * a.in_so;
+    const rk_i64 ob = a.out_off + (rk_i64)inst * a.out_so;
+    float2 zk[4], zr[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const rk_u64 k = k0 + (rk_u64)(e * RK_BATCH_THREADS + tid);
+        // k = N/2 reads Z[0]; lanes past the end read their own Z[k mod N/2]
+        // (a valid address) and discard it
+        const rk_u64 kk = k < half ? k : k % half;
+        const rk_u64 kr = kk ? half - kk : 0;
+        zk[e] = io.load(a, inst, kk, 0, zb + (rk_i64)kk);
+        zr[e] = io.load(a, inst, kr, 0, zb + (rk_i64)kr);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const rk_u64 k = k0 + (rk_u64)(e * RK_BATCH_THREADS + tid);
+        if (k > half) continue;
+        float2 x;
+        if (k == half)
+            x = make_float2(zk[e].x - zk[e].y, 0.0f);
+        else
+            x = rkfft::c_add(rkfft::c_mul(zk[e], a.chirp[k]), rkfft::c_mul(rkfft::c_conj(zr[e]), a.bspec[k]));
+        io.store(a, inst, k, 0, ob + (rk_i64)k, rkfft::c_scale(x, a.out_scale));
+    }
+}
+
 __device__ __forceinline__ void clear_counters(rk_u32* c, rk_u32 groups) {
     for (rk_u32 g = 0; g <= groups; ++g) __hip_atomic_store(c + g * RK_BATCH_LINE, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -109,7 +147,10 @@ extern "C" __global__ void __launch_bounds__(kThreads, RK_BATCH_MIN_WAVES) RK_BA
     a.in_sign = (inv && bp->item_first) ? -1.0f : 1.0f;
     a.out_sign = (inv && bp->item_last) ? -1.0f : 1.0f;
     a.out_scale = inv ? bp->inv_scale : bp->fwd_scale;
-    run_case(bp->kcase, a, lds, (rk_u64)(item - bp->first), tid);
+    if (bp->kcase == RK_BATCH_CASE_R2C)
+        r2c_combine_tile(a, (rk_u64)(item - bp->first), tid);
+    else
+        run_case(bp->kcase, a, lds, (rk_u64)(item - bp->first), tid);
     const rk_u32 groups = bp->groups;
     if (groups) {
         // every store of the tile has completed (they are device-coherent)

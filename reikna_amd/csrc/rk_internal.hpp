@@ -46,6 +46,7 @@ enum {
     VAR_TWIDDLE_KMUL = 1 | (1 << 2),
     VAR_TWIDDLE_CONJ_CHIRP = 1 | (2 << 2),
     VAR_BLUE = 2 | (2 << 2) | (1 << 4),
+    VAR_R2C = 1 << 6,   // batch kernel only: the real-to-complex combine pass
 };
 
 // All ahead-of-time kernels compiled into this library.

@@ -30,7 +30,14 @@ class BatchItem:
     """``count`` transforms of ``length`` complex64 elements; transform j of
     the item reads ``in[in_offset + j * in_dist + n]`` and writes
     ``out[out_offset + j * out_dist + k]`` (offsets and distances in
-    elements)."""
+    complex64 elements).
+
+    With ``r2c=True`` the item is numpy.fft.rfft of ``length`` (even, >= 4)
+    float32 values: transform j reads them from complex64 element
+    ``in_offset + j * in_dist`` on (as length/2 complex values) and writes
+    the length/2 + 1 non-negative frequencies; forward only, one launch with
+    the rest of the batch (a half-length complex FFT and the combine pass of
+    examples/demo_specialized_fft.py:156-331 inside the batch kernel)."""
 
     length: int
     count: int = 1
@@ -39,6 +46,17 @@ class BatchItem:
     in_dist: int = 0
     out_dist: int = 0
     scale: bool = False   # forward result multiplied by 1/N (inverse unscaled)
+    r2c: bool = False
+
+    @property
+    def in_length(self) -> int:
+        """complex64 elements one transform reads"""
+        return self.length // 2 if self.r2c else self.length
+
+    @property
+    def out_length(self) -> int:
+        """complex64 elements one transform writes"""
+        return self.length // 2 + 1 if self.r2c else self.length
 
 
 @dataclass(frozen=True)
@@ -64,16 +82,27 @@ class BatchOptions:
         return o
 
 
-def packed_items(lengths: Sequence[int], counts: Sequence[int] | None = None, scale: bool = False) -> list[BatchItem]:
-    """Items laid out back to back in one buffer (in and out alike)."""
+def packed_items(
+    lengths: Sequence[int],
+    counts: Sequence[int] | None = None,
+    scale: bool = False,
+    r2c: Sequence[bool] | bool = False,
+) -> list[BatchItem]:
+    """Items laid out back to back in one input and one output buffer
+    (``r2c``: per item or for all, real-to-complex items)."""
     counts = list(counts) if counts is not None else [1] * len(lengths)
     if len(counts) != len(lengths):
         raise ValueError("lengths and counts differ in size")
-    items, off = [], 0
-    for n, c in zip(lengths, counts):
+    kinds = [bool(r2c)] * len(lengths) if isinstance(r2c, bool) else [bool(x) for x in r2c]
+    if len(kinds) != len(lengths):
+        raise ValueError("lengths and r2c differ in size")
+    items, ioff, ooff = [], 0, 0
+    for n, c, real in zip(lengths, counts, kinds):
         n, c = int(n), int(c)
-        items.append(BatchItem(n, c, off, off, n, n, scale))
-        off += n * c
+        ni, no = (n // 2, n // 2 + 1) if real else (n, n)
+        items.append(BatchItem(n, c, ioff, ooff, ni, no, scale, real))
+        ioff += ni * c
+        ooff += no * c
     return items
 
 
@@ -93,7 +122,7 @@ class BatchFFT:
             arr[k].out_offset = int(it.out_offset)
             arr[k].in_dist = int(it.in_dist)
             arr[k].out_dist = int(it.out_dist)
-            arr[k].flags = _lib.RK_BATCH_SCALE if it.scale else 0
+            arr[k].flags = (_lib.RK_BATCH_SCALE if it.scale else 0) | (_lib.RK_BATCH_R2C if it.r2c else 0)
         if not self.virtual:
             device.activate()
         opts = self.options.to_c(dry=self.virtual)
@@ -105,10 +134,11 @@ class BatchFFT:
         self.workspace_bytes = int(sz.value)
         self._ws: Buffer | None = None
         # elements each side must hold
-        self.in_elements = max((it.in_offset + (it.count - 1) * it.in_dist + it.length for it in self.items if it.count),
-                               default=0)
+        self.in_elements = max(
+            (it.in_offset + (it.count - 1) * it.in_dist + it.in_length for it in self.items if it.count), default=0
+        )
         self.out_elements = max(
-            (it.out_offset + (it.count - 1) * it.out_dist + it.length for it in self.items if it.count), default=0
+            (it.out_offset + (it.count - 1) * it.out_dist + it.out_length for it in self.items if it.count), default=0
         )
 
     @classmethod
@@ -146,13 +176,21 @@ class BatchFFT:
         return self._ws.ptr
 
     def __call__(self, queue: Queue, output: Array, input: Array, inverse: int = 0) -> None:
+        """Runs the batch.  ``output`` is complex64; ``input`` is complex64, or
+        float32 (two values per complex64 element of the item offsets: the
+        natural buffer of a batch of real-to-complex items)."""
         if self.virtual:
             raise ValueError("a batch planned for a VirtualDevice cannot execute")
-        for name, arr, need in (("output", output, self.out_elements), ("input", input, self.in_elements)):
-            if arr.dtype.itemsize != 8 or arr.dtype.kind != "c":
-                raise ValueError(f"{name} must be complex64")
-            if arr.size < need:
-                raise ValueError(f"{name} holds {arr.size} elements, the batch needs {need}")
+        if output.dtype.itemsize != 8 or output.dtype.kind != "c":
+            raise ValueError("output must be complex64")
+        if output.size < self.out_elements:
+            raise ValueError(f"output holds {output.size} elements, the batch needs {self.out_elements}")
+        real_in = input.dtype.kind == "f" and input.dtype.itemsize == 4
+        if not real_in and (input.dtype.itemsize != 8 or input.dtype.kind != "c"):
+            raise ValueError("input must be complex64 or float32")
+        have = input.size // 2 if real_in else input.size
+        if have < self.in_elements:
+            raise ValueError(f"input holds {have} complex64 elements' worth, the batch needs {self.in_elements}")
         _lib.call(
             "rk_batch_execute", self._h, ctypes.c_void_p(output.data_ptr), ctypes.c_void_p(input.data_ptr),
             int(bool(inverse)), ctypes.c_void_p(self._workspace() or None), queue.stream,

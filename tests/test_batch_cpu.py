@@ -118,3 +118,36 @@ def test_order_modes_keep_dependencies(vdev):
     for order in (0, 1, 2):
         i = info(vdev, lengths, order=order)
         assert i["launches"] == 1 and i["levels"] >= 4
+
+
+def test_r2c_items(vdev):
+    from reikna_amd.fft.batch import packed_items as pk
+
+    # half-length chain + one combine pass
+    assert info(vdev, [1024])["levels"] == 1
+    i = BatchFFT(vdev, pk([1024], r2c=True)).info
+    assert i["levels"] == 2 and i["passes"] == 2
+    i = BatchFFT(vdev, pk([131072], r2c=True)).info      # 65536 = 256 * 256, then the combine
+    assert i["levels"] == 3 and i["passes"] == 3
+    i = BatchFFT(vdev, pk([1994], r2c=True)).info        # 997 (prime): single-kernel Bluestein, then the combine
+    assert i["levels"] == 2
+    # the half-length spectrum lives in the workspace
+    assert BatchFFT(vdev, pk([1024], [3], r2c=True)).workspace_bytes >= 3 * 512 * 8
+    # mixed with complex items: still one launch
+    i = BatchFFT(vdev, pk([1024, 997, 8192], r2c=[True, False, True])).info
+    assert i["launches"] == 1
+
+
+@pytest.mark.parametrize("n", [2, 5, 1001])
+def test_r2c_bad_lengths(vdev, n):
+    from reikna_amd.fft.batch import packed_items as pk
+
+    with pytest.raises(Exception, match="even"):
+        BatchFFT(vdev, pk([n], r2c=True))
+
+
+def test_r2c_grouped_refused(vdev):
+    from reikna_amd.fft.batch import packed_items as pk
+
+    with pytest.raises(Exception, match="queue mode"):
+        BatchFFT(vdev, pk([64], r2c=True), BatchOptions(mode="grouped"))

@@ -166,3 +166,91 @@ def test_autotune_small(queue):
     data = packed_data(lengths, [1] * len(lengths), seed=13)
     res, _ = run_batch(queue, items, data, data.size, options=best)
     check_items(items, data, res)
+
+
+# ---------------------------------------------------------------------------
+# real-to-complex items (numpy.fft.rfft) in the same launch as complex ones
+# ---------------------------------------------------------------------------
+R2C_LENGTHS = [4, 6, 10, 64, 1000, 1024, 1994, 6000, 8192, 24576, 131072]
+
+
+def r2c_batch_data(items, seed):
+    """One float32 buffer: complex items' values as (re, im) pairs, real
+    items' values as they are (item offsets count complex64 elements)."""
+    total = max(it.in_offset + (it.count - 1) * it.in_dist + it.in_length for it in items)
+    rng = numpy.random.default_rng(seed)
+    return rng.uniform(-1, 1, 2 * total).astype(numpy.float32)
+
+
+def check_r2c_items(items, flat_in, host_out):
+    cin = flat_in.view(numpy.complex64)
+    worst = 0.0
+    for it in items:
+        for j in range(it.count):
+            o = it.in_offset + j * it.in_dist
+            y = host_out[it.out_offset + j * it.out_dist: it.out_offset + j * it.out_dist + it.out_length]
+            if it.r2c:
+                x = flat_in[2 * o: 2 * o + it.length].astype(numpy.float64)
+                ref = numpy.fft.rfft(x)
+            else:
+                ref = oracle.fftn(cin[o: o + it.length], (0,))
+            if it.scale:
+                ref = ref / it.length
+            err = oracle.rel_l2(y, ref)
+            tol = oracle.tolerance((it.length,))
+            assert err <= tol, f"length {it.length} r2c={it.r2c} (copy {j}): rel-L2 {err:.3g} > {tol:.3g}"
+            worst = max(worst, err / tol)
+    return worst
+
+
+@pytest.mark.parametrize("scale", [False, True])
+def test_r2c_items_in_mixed_batch(queue, scale):
+    rng = numpy.random.default_rng(31)
+    c2c = sample_lengths(12, seed=16) + [997, 65521]
+    lengths = R2C_LENGTHS + c2c
+    kinds = [True] * len(R2C_LENGTHS) + [False] * len(c2c)
+    counts = [int(c) for c in rng.integers(1, 4, len(lengths))]
+    items = packed_items(lengths, counts, scale=scale, r2c=kinds)
+    flat = r2c_batch_data(items, seed=32)
+    batch = BatchFFT(queue.device, items)
+    inp = Array.from_host(queue, flat)   # float32 input
+    n_out = max(it.out_offset + (it.count - 1) * it.out_dist + it.out_length for it in items)
+    out = Array.from_host(queue, numpy.zeros(n_out, numpy.complex64))
+    batch(queue, out, inp)
+    res = out.get(queue)
+    batch.check()
+    check_r2c_items(items, flat, res)
+    assert batch.info["launches"] == 1
+
+
+def test_r2c_alone_equals_in_mix(queue):
+    n = 24576
+    alone_items = packed_items([n], [2], r2c=True)
+    flat = r2c_batch_data(alone_items, seed=33)
+    b1 = BatchFFT(queue.device, alone_items)
+    inp = Array.from_host(queue, flat)
+    out = Array.from_host(queue, numpy.zeros(2 * (n // 2 + 1), numpy.complex64))
+    b1(queue, out, inp)
+    alone = out.get(queue)
+    # the same item at other offsets inside a mixed batch
+    mix = packed_items([1000, n, 65521, 8192], [3, 2, 1, 2], r2c=[False, True, False, True])
+    it = mix[1]
+    flat2 = r2c_batch_data(mix, seed=34)
+    flat2[2 * it.in_offset: 2 * it.in_offset + 2 * n] = flat[: 2 * n]
+    b2 = BatchFFT(queue.device, mix)
+    inp2 = Array.from_host(queue, flat2)
+    n_out = max(i.out_offset + (i.count - 1) * i.out_dist + i.out_length for i in mix)
+    out2 = Array.from_host(queue, numpy.zeros(n_out, numpy.complex64))
+    b2(queue, out2, inp2)
+    res = out2.get(queue)
+    assert numpy.array_equal(res[it.out_offset: it.out_offset + 2 * (n // 2 + 1)], alone)
+    check_r2c_items(mix, flat2, res)
+
+
+def test_r2c_forward_only(queue):
+    items = packed_items([64, 128], r2c=[True, False])
+    batch = BatchFFT(queue.device, items)
+    inp = Array.from_host(queue, r2c_batch_data(items, seed=35))
+    out = Array.from_host(queue, numpy.zeros(200, numpy.complex64))
+    with pytest.raises(Exception, match="forward only"):
+        batch(queue, out, inp, 1)
