@@ -499,8 +499,8 @@ static rk_u32 xcd_count() {
 // neighbours: each group of 32 / C consecutive tiles is sent to one XCD, so
 // the neighbours meet in that XCD's L2 instead of each fetching whole lines
 // from HBM (RK_XCD_GROUP overrides the group size; 1 turns it off).  Measured
-// on the two 1024-point passes of a 64 x 2^20 transform with C = 4: 941 us
-// without groups, 683 us with groups of 4, 632 us with groups of 8.
+// on 64 x 2^20 as two 1024-point passes with C = 4: 953 us without groups,
+// 715 us with groups of 4, 669 us with groups of 8.
 static void set_xcd_group(PassPlan& pp) {
     static const int forced = [] {
         const char* e = getenv("RK_XCD_GROUP");
