@@ -230,3 +230,14 @@ def test_strided_input_and_output(queue):
     fftc = fft.compile(queue.device)
     fftc(queue, out, dev)
     assert oracle.rel_l2(out.get(queue), oracle.fftn(data, (1,))) <= oracle.tolerance((64, 1024), (1,))
+
+
+@pytest.mark.parametrize("shape", [(4096, 40), (2048, 100), (16384, 5), (3 * 2**12, 7), (1024, 3, 5)])
+def test_long_strided_axis(queue, shape):
+    """Axis 0 long and strided: column tiles narrower than 16 instances, sent to
+    one XCD in groups of consecutive tiles; above 4096 points two passes."""
+    data = oracle.test_array(shape, seed=sum(shape))
+    for inverse in (False, True):
+        res, _ = run_fft(queue, data, (0,), inverse=inverse, inplace=False)
+        ref = oracle.ifftn(data, (0,)) if inverse else oracle.fftn(data, (0,))
+        assert oracle.rel_l2(res, ref) <= oracle.tolerance(shape, (0,))
