@@ -342,7 +342,9 @@ typedef struct {
 
 int rk_server_create(const rk_server_config* config, rk_server** server);
 /* Queue requests (they wait for memory and a slot); RK_ERR_BUDGET for a
-   request that can never fit in the budget. */
+   request that can never fit in the budget, RK_ERR_INVALID for an id that is
+   still in use (submitted and not released).  Requests before the failing
+   one stay queued. */
 int rk_server_submit(rk_server* server, const rk_request* requests, int n);
 /* Reap finished launches, admit and launch queued requests, and return up to
    max_done completions (never blocks). */
@@ -353,7 +355,8 @@ uint64_t rk_server_now_ns(void);
 /* Open-loop replay: submit each request when rk_server_now_ns() reaches its
    t_arrival_ns, progress the server meanwhile, and write the n completions to
    done (in completion order); outputs are released as they complete unless
-   keep_outputs.  Fails if nothing completes for stall_ms (0: 10 s). */
+   keep_outputs.  Fails if nothing completes for stall_ms (0: 10 s), and
+   refuses to start while the server has other requests queued or in flight. */
 int rk_server_replay(rk_server* server, const rk_request* requests, int n,
                      rk_completion* done, int keep_outputs, int stall_ms);
 int rk_server_destroy(rk_server* server);

@@ -32,6 +32,7 @@
 #include <memory>
 #include <tuple>
 #include <unordered_map>
+#include <unordered_set>
 
 #include "rk_batch_internal.hpp"
 
@@ -145,6 +146,7 @@ struct rk_server {
     std::deque<Request> pending;
     std::vector<Slot> slots;
     std::unordered_map<uint64_t, std::pair<size_t, size_t>> outputs;   // id -> (offset, bytes)
+    std::unordered_set<uint64_t> live;   // ids submitted and not yet released
     std::vector<rk_completion> finished;
     uint64_t launches = 0, completed = 0;
 };
@@ -269,7 +271,11 @@ extern "C" int rk_server_submit(rk_server* s, const rk_request* reqs, int n) {
         if (need > s->arena.size())
             return fail(RK_ERR_BUDGET, "rk_server_submit: request %d needs %zu bytes, the budget is %zu", i, need,
                         s->arena.size());
+        if (s->live.count(r.id))
+            return fail(RK_ERR_INVALID, "rk_server_submit: request %d: id %llu is still in use", i,
+                        (unsigned long long)r.id);
         if (q.r.t_arrival_ns == 0) q.r.t_arrival_ns = now_ns();
+        s->live.insert(r.id);
         s->pending.push_back(q);
     }
     return RK_OK;
@@ -430,6 +436,7 @@ extern "C" int rk_server_release(rk_server* s, uint64_t id) {
     if (it == s->outputs.end()) return fail(RK_ERR_INVALID, "rk_server_release: unknown request %llu", (unsigned long long)id);
     s->arena.release(it->second.first, it->second.second);
     s->outputs.erase(it);
+    s->live.erase(id);
     return RK_OK;
 }
 
@@ -460,6 +467,10 @@ extern "C" uint64_t rk_server_now_ns(void) { return now_ns(); }
 extern "C" int rk_server_replay(rk_server* s, const rk_request* reqs, int n, rk_completion* done, int keep_outputs,
                                 int stall_ms) {
     if (!s || n < 0 || (n > 0 && (!reqs || !done))) return fail(RK_ERR_INVALID, "rk_server_replay: bad arguments");
+    // every completion it sees must be one of `reqs`
+    bool idle = s->pending.empty() && s->finished.empty();
+    for (const auto& sl : s->slots) idle = idle && !sl.busy;
+    if (!idle) return fail(RK_ERR_INVALID, "rk_server_replay: the server has requests of its own in flight");
     int next = 0, completed = 0;
     std::vector<rk_completion> buf(1024);
     uint64_t last_progress = now_ns();

@@ -132,3 +132,27 @@ def test_stream_reports_latency(queue):
     assert 0 < res.p50_ms <= res.p99_ms
     assert res.throughput_rps > 0 and math.isfinite(res.gflops)
     server.close()
+
+
+def test_ids_in_use_and_busy_replay_refused(queue):
+    specs = make_specs(queue, [((1024,), 2), ((256,), 1)], seed=29)
+    server = FFTServer(queue.device, budget_bytes=64 * 2**20)
+    server.submit([(0, specs[0], 0)])
+    with pytest.raises(Exception, match="still in use"):
+        server.submit([(0, specs[1], 0)])
+    # a native replay needs an idle server (its completions are all its own)
+    with pytest.raises(Exception, match="in flight"):
+        run_stream(server, specs, rate=1000.0)
+    drain_ids = 0
+    while drain_ids < 1:
+        for c in server.progress():
+            check_output(server, queue, c, specs[0], {})
+            server.release(c.id)
+            drain_ids += 1
+    server.submit([(0, specs[1], 0)])   # id free again after release
+    while not server.progress():
+        pass
+    server.release(0)
+    res = run_stream(server, specs, rate=1000.0)
+    assert res.completed == 2
+    server.close()
