@@ -218,7 +218,8 @@ def main(argv=None) -> int:
             "shape": list(shape),
             "axes": list(axes),
             "distribution": f"{dist}, seed {seed}",
-            "passes": [{"L": p.L, "C": p.C, "load": p.load_mode, "store": p.store_mode} for p in passes],
+            "passes": [{"L": p.L, "C": p.C, "load": p.load_mode, "store": p.store_mode, "stream": p.stream}
+                       for p in passes],
         },
         "roofline": {
             "bound": "hbm",

@@ -146,6 +146,7 @@ class FftPassInfo(ctypes.Structure):
         ("jit", ctypes.c_int),
         ("variant", ctypes.c_int),
         ("in_dims", ctypes.c_int64 * 3),
+        ("stream", ctypes.c_int),
     ]
 
 

@@ -131,6 +131,8 @@ VAR_CHIRP = 2                        # first pass of a multi-pass Bluestein chai
 VAR_TWIDDLE_KMUL = 1 | (1 << 2)      # last pass of the forward Bluestein chain
 VAR_TWIDDLE_CONJ_CHIRP = 1 | (2 << 2)  # last pass of the second Bluestein chain
 VAR_BLUE = 2 | (2 << 2) | (1 << 4)   # single-kernel Bluestein
+VAR_STREAM = 1 << 7                  # catalogue kernel with nontemporal loads/stores (StreamIO)
+STREAM_MAX = 4096                    # streaming kernels for plain/twiddle passes up to this length
 
 
 def catalog_entries() -> list[tuple[int, int, int]]:
@@ -143,6 +145,10 @@ def catalog_entries() -> list[tuple[int, int, int]]:
         if 16 <= length <= 4096:
             for v in (VAR_TWIDDLE, VAR_CHIRP, VAR_TWIDDLE_KMUL, VAR_TWIDDLE_CONJ_CHIRP):
                 ents.add((length, tile_c_col(length), v))
+    # streaming twins of the passes of large transforms
+    for length, c, v in list(ents):
+        if length <= STREAM_MAX and v in (VAR_PLAIN, VAR_TWIDDLE):
+            ents.add((length, c, v | VAR_STREAM))
     return sorted(ents)
 
 
@@ -212,6 +218,7 @@ def generate_batch_cases(out_dir: Path) -> list[Path]:
 
 
 def variant_cost(v: int) -> float:
+    v &= ~VAR_STREAM
     return 2.0 if v & (1 << 4) else 1.3 if v else 1.0
 
 
@@ -233,7 +240,12 @@ def generate_catalog(out_dir: Path) -> list[Path]:
             f"namespace rk_catalog_{k} {{",
             "template <int L, int C, int V>",
             "__global__ void __launch_bounds__((rkfft::pass_threads<L, C>()))",
-            "rk_fft_pass(rk_pass_args a) { rkfft::fft_pass_body<L, C, V>(a, rkfft::PlainIO{}); }",
+            "rk_fft_pass(rk_pass_args a) {",
+            "    if constexpr ((V & RK_VAR_STREAM) != 0)",
+            "        rkfft::fft_pass_body<L, C, (V & ~RK_VAR_STREAM)>(a, rkfft::StreamIO{});",
+            "    else",
+            "        rkfft::fft_pass_body<L, C, V>(a, rkfft::PlainIO{});",
+            "}",
             "static const rk::KernelEntry entries[] = {",
         ]
         for length, c, v in sorted(part):

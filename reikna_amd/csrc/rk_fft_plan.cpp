@@ -517,7 +517,36 @@ static void set_xcd_group(PassPlan& pp) {
     }
 }
 
+// Bytes above which a plain or twiddle pass uses the streaming kernel: the
+// data it reads and writes (one buffer when in place) then exceeds the
+// 256 MB Infinity Cache, so nothing it touches would be read from a cache
+// again (RK_STREAM_MIN_BYTES overrides; 0 turns streaming off).
+static double stream_min_bytes() {
+    static const double v = [] {
+        const char* e = getenv("RK_STREAM_MIN_BYTES");
+        return e && *e ? atof(e) : 256.0 * 1024 * 1024;
+    }();
+    return v;
+}
+
+static bool stream_pass(const PassPlan& pp) {
+    if (pp.variant != VAR_PLAIN && pp.variant != VAR_TWIDDLE) return false;
+    if (stream_min_bytes() <= 0) return false;
+    const double bytes = 8.0 * (double)pp.args.n_inst * (double)pp.L * (pp.src == pp.dst ? 1.0 : 2.0);
+    return bytes > stream_min_bytes();
+}
+
 static int find_kernel(PassPlan& pp, uint32_t flags) {
+    if (pp.stream) {
+        for (const KernelEntry& e : catalog())
+            if (e.L == pp.L && e.C == pp.C && e.variant == (pp.variant | VAR_STREAM)) {
+                pp.aot = e.func;
+                pp.threads = e.threads;
+                pp.lds_bytes = e.lds_bytes;
+                return RK_OK;
+            }
+        pp.stream = false;   // no streaming twin: the plain kernel
+    }
     for (const KernelEntry& e : catalog())
         if (e.L == pp.L && e.C == pp.C && e.variant == pp.variant) {
             pp.aot = e.func;
@@ -726,6 +755,7 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
     size_t ws_elems = 0;
     bool two_ws = false;
     PassHook hook = [&](PassPlan& pp, const AxisPass& ap, int64_t n) -> int {
+        pp.stream = stream_pass(pp);
         int rc = find_kernel(pp, d->flags);
         if (rc != RK_OK) return rc;
         set_xcd_group(pp);
@@ -962,6 +992,7 @@ extern "C" int rk_fft_plan_pass_info(const rk_fft_plan* plan, int pass, rk_fft_p
     info->lds_bytes = pp.lds_bytes;
     info->jit = pp.jit ? 1 : 0;
     info->variant = pp.variant;
+    info->stream = pp.stream ? 1 : 0;
     info->in_dims[0] = (int64_t)(pp.args.n_inst / (pp.args.dim_I * pp.args.dim_NS * pp.args.dim_H));
     info->in_dims[1] = (int64_t)pp.args.n_valid_in;
     info->in_dims[2] = (int64_t)pp.args.dim_I;

@@ -47,6 +47,7 @@ enum {
     VAR_TWIDDLE_CONJ_CHIRP = 1 | (2 << 2),
     VAR_BLUE = 2 | (2 << 2) | (1 << 4),
     VAR_R2C = 1 << 6,   // batch kernel only: the real-to-complex combine pass
+    VAR_STREAM = 1 << 7,   // catalogue entries: the kernel uses StreamIO (RK_VAR_STREAM)
 };
 
 // All ahead-of-time kernels compiled into this library.
@@ -114,6 +115,7 @@ struct PassPlan {
     hipFunction_t jit = nullptr;    // run-time compiled kernel
     bool user_kernel = false;       // replaced with rk_fft_plan_set_pass_kernel
     bool is_first = false, is_last = false;
+    bool stream = false;            // nontemporal loads/stores (StreamIO kernel)
     uint64_t grid = 0;
     int axis = 0;
     int kind = 0;

@@ -37,6 +37,7 @@ class PassInfo:
     lds_bytes: int
     jit: bool
     variant: int
+    stream: bool = False   # nontemporal loads/stores (transform larger than the Infinity Cache)
 
 
 def _element_layout(meta: ArrayMetadata) -> tuple[list[int], int]:
@@ -106,6 +107,7 @@ class NativeFftPlan:
                     src=BUFFERS[info.src], dst=BUFFERS[info.dst], axis=info.axis, pre_ops=info.pre_ops,
                     post_ops=info.post_ops, n_inst=info.n_inst, grid=info.grid, axis_len=info.axis_len,
                     Ns=info.Ns, lds_bytes=info.lds_bytes, jit=bool(info.jit), variant=info.variant,
+                    stream=bool(info.stream),
                 )
             )
         return out
