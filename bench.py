@@ -51,7 +51,7 @@ WORKLOADS = {
 # traffic = 2 * FETCH_SIZE + WRITE_SIZE (KiB per dispatch).
 MEASURED_TRAFFIC = {
     # (workload, L, C, variant of the dominant pass): bytes per launch
-    ("c2b", 128, 32, 1): (2 * 262289.8 + 524288.0) * 1024,
+    ("c2b", 128, 32, 1): (2 * 262287.0 + 524290.6) * 1024,
     # (workload, kernel) for the one-launch batch (profiles/fft_batch_c4_512_uneven.md)
     ("c4", "rk_fft_batch_blue"): (2 * 132907.8 + 204343.6) * 1024,
 }
