@@ -109,7 +109,7 @@ def main(argv=None) -> int:
     ap.add_argument("--workload", default="c2b", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--autotune", action="store_true", help="c4: pick batch planner knobs by measurement")
-    ap.add_argument("--requests", type=int, default=400, help="c5: requests per rate point")
+    ap.add_argument("--requests", type=int, default=1000, help="c5: requests per rate point")
     ap.add_argument("--rates", default="1000,2000,5000,10000,20000,50000", help="c5: offered rates (requests/s)")
     ap.add_argument("--small-rates", default="20000,50000,100000,200000,400000,800000",
                     help="c5: offered rates of the small-request stream")
