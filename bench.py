@@ -43,7 +43,14 @@ WORKLOADS = {
     # request serving: handled by bench_serve
     "c5": (None, None, "normal", 8, "Poisson stream of c64 FFT requests: 1-D 2^8..2^20 and 2-D 128^2..2048^2, "
            "batch 1..64 (log-uniform, <= 2^25 elements), arrivals seed 7"),
+    # the computations beside the FFT (transpose, real-input transforms, shift, spectrogram)
+    "transpose": ((8192, 8192), None, "normal", 11, "c64 Transpose 8192 x 8192"),
+    "rfft": ((64, 2**20), (1,), "normal", 12, "f32 RFFT 64 x 2^20 over axis 1"),
+    "irfft": ((64, 2**19 + 1), (1,), "normal", 13, "c64 IRFFT 64 x (2^19 + 1) over axis 1"),
+    "fftshift": ((8192, 8192), (0, 1), "normal", 14, "c64 FFTShift 8192 x 8192 in place"),
+    "spectrogram": ((2**24,), None, "normal", 15, "f32 Spectrogram of 2^24 samples, 256-point frames, overlap 128"),
 }
+COMPUTATIONS = ("transpose", "rfft", "irfft", "fftshift", "spectrogram")
 
 # HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE
 # and WRITE_SIZE runs of this benchmark (profiles/fft_pass_c2b_64x2p20.md):
@@ -133,6 +140,8 @@ def main(argv=None) -> int:
         return bench_batch(args)
     if args.workload == "c5":
         return bench_serve(args)
+    if args.workload in COMPUTATIONS:
+        return bench_computation(args)
     shape, axes, dist, seed, desc = WORKLOADS[args.workload]
     dev = Device(0)
     queue = Queue(dev)
@@ -383,6 +392,132 @@ def bench_batch(args) -> int:
         }
     print(json.dumps(out_json), flush=True)
     return 0 if worst <= 1.0 and r2c["worst_rel_l2_over_bound"] <= 1.0 else 1
+
+
+def bench_computation(args) -> int:
+    """One of the computations beside the FFT at a size that streams through
+    HBM: the whole computation (all its launches) timed with HIP events,
+    checked against numpy in FP64, priced by its algorithmic bytes."""
+    import numpy
+
+    from reikna_amd import Array, Event, Queue
+    from reikna_amd.algorithms import Transpose
+    from reikna_amd.fft import FFTShift
+    from reikna_amd.fft.rfft import IRFFT, RFFT
+    from reikna_amd.fft.spectrogram import Spectrogram
+    from reikna_amd.runtime import Device
+
+    name = args.workload
+    shape, axes, dist, seed, desc = WORKLOADS[name]
+    dev = Device(0)
+    queue = Queue(dev)
+    rng = numpy.random.default_rng(seed)
+    print(f"[bench] {desc} on {dev.name}", file=sys.stderr, flush=True)
+    flops = None
+    inplace = False
+    if name == "transpose":
+        host = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(numpy.complex64)
+        comp = Transpose(host)
+        ref_fn = lambda x: x.T  # noqa: E731
+        out_shape, out_dtype = shape[::-1], numpy.complex64
+        alg_bytes = 16.0 * host.size
+        tol = 0.0
+    elif name == "rfft":
+        host = rng.standard_normal(shape).astype(numpy.float32)
+        comp = RFFT(host)
+        ref_fn = lambda x: numpy.fft.rfft(x.astype(numpy.float64), axis=-1)  # noqa: E731
+        out_shape, out_dtype = (shape[0], shape[1] // 2 + 1), numpy.complex64
+        alg_bytes = 4.0 * host.size + 8.0 * shape[0] * (shape[1] // 2 + 1)
+        flops = 2.5 * host.size * math.log2(shape[1])
+        tol = 2e-7 * math.log2(shape[1])
+    elif name == "irfft":
+        n = (shape[1] - 1) * 2
+        real = rng.standard_normal((shape[0], n))
+        host = numpy.fft.rfft(real, axis=-1).astype(numpy.complex64)
+        comp = IRFFT(host)
+        ref_fn = lambda x: numpy.fft.irfft(x.astype(numpy.complex128), axis=-1)  # noqa: E731
+        out_shape, out_dtype = (shape[0], n), numpy.float32
+        alg_bytes = 8.0 * host.size + 4.0 * shape[0] * n
+        flops = 2.5 * shape[0] * n * math.log2(n)
+        tol = 2e-7 * math.log2(n)
+    elif name == "fftshift":
+        host = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(numpy.complex64)
+        comp = FFTShift(host, axes=axes)
+        ref_fn = lambda x: numpy.fft.fftshift(x, axes=axes)  # noqa: E731
+        out_shape, out_dtype = shape, numpy.complex64
+        alg_bytes = 16.0 * host.size
+        tol = 0.0
+        inplace = True
+    else:   # spectrogram
+        host = rng.standard_normal(shape).astype(numpy.float32)
+        comp = Spectrogram(host, fft_size=256, noverlap=128)
+        frames = (shape[0] - 128) // 128
+        out_shape, out_dtype = (129, frames), numpy.float32
+
+        def ref_fn(x):
+            x = x.astype(numpy.float64)
+            idx = numpy.arange(256)[None, :] + 128 * numpy.arange(frames)[:, None]
+            win = 0.5 * (1 - numpy.cos(2 * numpy.pi * numpy.arange(256) / 255))
+            return numpy.abs(numpy.fft.rfft(x[idx] * win, axis=-1)).T
+
+        alg_bytes = 4.0 * host.size + 4.0 * 129 * frames
+        flops = 5.0 * frames * 256 * 8
+        tol = 2e-7 * 8
+    compiled = comp.compile(dev)
+    inp = Array.from_host(queue, host)
+    out = inp if inplace else Array.empty(dev, out_shape, out_dtype)
+
+    def step():
+        compiled(queue, out, inp)
+
+    for _ in range(args.warmup):
+        step()
+    queue.synchronize()
+    e0, e1 = Event(), Event()
+    e0.record(queue)
+    for _ in range(args.steps):
+        step()
+    e1.record(queue)
+    queue.synchronize()
+    ms = e0.elapsed_ms(e1) / args.steps
+    # check: one more call from a fresh input
+    inp.set(queue, host)
+    step()
+    res = out.get(queue)
+    ref = ref_fn(host)
+    if tol == 0.0:
+        err = 0.0 if numpy.array_equal(res, ref.astype(res.dtype)) else float("inf")
+    else:
+        err = float(numpy.linalg.norm(res - ref) / numpy.linalg.norm(ref))
+    achieved = alg_bytes / (ms / 1e3) / 1e9
+    out_json = {
+        "metric": METRIC,
+        "value": round(flops / (ms / 1e3) / 1e9, 1) if flops else round(achieved, 1),
+        "unit": "GFLOP/s" if flops else "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64" if out_dtype == numpy.complex64 or host.dtype == numpy.complex64 else "f32",
+        "data": "synthetic",
+        "config": {"workload": desc, "shape": list(shape), "distribution": f"{dist}, seed {seed}"},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "whole computation (all its launches)",
+        },
+        "rel_l2": err,
+        "rel_l2_bound": tol,
+    }
+    print(json.dumps(out_json), flush=True)
+    return 0 if err <= tol else 1
 
 
 def bench_batch_r2c(queue, lengths, timed, steps) -> dict:
