@@ -298,7 +298,7 @@ int rk_batch_plan_stats(const rk_batch_plan* plan, int64_t* work_items,
 int rk_batch_plan_info(const rk_batch_plan* plan, rk_batch_info* info);
 /* Synchronous: 1 in *hit if a dependency wait of an earlier execute gave up
    after its poll limit (the results of that execute are not valid); clears
-   the flag. */
+   the flag and the plan's dependency counters. */
 int rk_batch_plan_check(rk_batch_plan* plan, int* hit);
 
 /* ------------------------------------------------------------------ */
@@ -318,6 +318,10 @@ typedef struct {
     int64_t plain_min_elements;  /* RK_SERVER_BATCHED: requests of at least this many
                                     elements run by themselves with their single-shape
                                     plan (0: never) */
+    uint32_t spin_limit;   /* polls before a dependency wait of the batch kernel gives
+                              up and flags its launch (0: 2^24; small values are for
+                              tests of the failure path) */
+    uint32_t _reserved;
 } rk_server_config;
 
 typedef struct {
@@ -334,11 +338,15 @@ typedef struct {
     uint64_t id;
     void* out;             /* device output, valid until rk_server_release(id) */
     uint64_t t_arrival_ns, t_admitted_ns, t_done_ns;
+    int32_t status;        /* 0; 1 if a dependency wait of its launch gave up (the
+                              output is not valid; the server has reset its state) */
+    int32_t _reserved;
 } rk_completion;
 
 typedef struct {
     int64_t pending, in_flight_requests, in_flight_launches, completed, launches, templates;
     size_t budget_used, budget_bytes;
+    int64_t failed_launches;   /* launches whose completions carry status 1 */
 } rk_server_stats_t;
 
 int rk_server_create(const rk_server_config* config, rk_server** server);

@@ -40,6 +40,8 @@ class ServerConfig(ctypes.Structure):
         ("max_requests", ctypes.c_int32),
         ("mode", ctypes.c_uint32),
         ("plain_min_elements", ctypes.c_int64),
+        ("spin_limit", ctypes.c_uint32),
+        ("_reserved", ctypes.c_uint32),
     ]
 
 
@@ -62,6 +64,8 @@ class Completion(ctypes.Structure):
         ("t_arrival_ns", ctypes.c_uint64),
         ("t_admitted_ns", ctypes.c_uint64),
         ("t_done_ns", ctypes.c_uint64),
+        ("status", ctypes.c_int32),
+        ("_reserved", ctypes.c_int32),
     ]
 
 
@@ -75,6 +79,7 @@ class ServerStats(ctypes.Structure):
         ("templates", ctypes.c_int64),
         ("budget_used", ctypes.c_size_t),
         ("budget_bytes", ctypes.c_size_t),
+        ("failed_launches", ctypes.c_int64),
     ]
 
 

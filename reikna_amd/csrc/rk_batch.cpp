@@ -324,6 +324,7 @@ struct rk_batch_plan {
     int launches = 0;
     bool dry = false;
     bool r2c = false;                    // has real-to-complex items (forward only)
+    size_t counter_bytes = 0;            // dependency counters (work.counters)
 };
 
 extern "C" int rk_batch_plan_destroy(rk_batch_plan* plan) {
@@ -461,6 +462,8 @@ static int create_queue(const rk_batch_item* items, int n_items, rk_batch_plan* 
     plan->work.n_passes = (rk_u32)queued.size();
     plan->work.total = (rk_u32)total;
     plan->work.spin_limit = kSpinLimit;
+    if (const char* e = getenv("RK_BATCH_SPIN_LIMIT"))   // tests of the failure path
+        if (*e && atol(e) > 0) plan->work.spin_limit = (uint32_t)atol(e);
     plan->work.poll_sleep_max = rk::batch_poll_sleep_max();
     size_t lines = 0;
     if (plan->dry) {
@@ -476,6 +479,7 @@ static int create_queue(const rk_batch_item* items, int n_items, rk_batch_plan* 
     for (auto& qp : queued) rk::resolve_tables(qp.a, qp.cp->pp, nullptr);
     rk::assemble_queue(queued, plan->opt.order, plan->flavour, plan->passes, plan->tile_pass, plan->levels, lines);
     const size_t off_state = off_cnt + align(lines * RK_BATCH_LINE * sizeof(rk_u32));
+    plan->counter_bytes = off_state - off_cnt;
     const size_t bytes = off_state + 256;
     RK_HIP(hipMalloc(&plan->dev, bytes));
     char* base = (char*)plan->dev;
@@ -610,8 +614,11 @@ extern "C" int rk_batch_plan_check(rk_batch_plan* plan, int* hit) {
     RK_HIP(hipDeviceSynchronize());
     RK_HIP(hipMemcpy(&flag, plan->work.state + 2, sizeof(flag), hipMemcpyDeviceToHost));
     if (flag) {
-        const rk_u32 zero = 0;
-        RK_HIP(hipMemcpy(plan->work.state + 2, &zero, sizeof(zero), hipMemcpyHostToDevice));
+        // the counters of the failed execute may not have returned to zero:
+        // clear them with the flag (the device is idle after the sync above)
+        RK_HIP(hipMemset(plan->work.counters, 0, plan->counter_bytes));
+        RK_HIP(hipMemset(plan->work.state, 0, 256));
+        RK_HIP(hipDeviceSynchronize());
     }
     *hit = flag ? 1 : 0;
     return RK_OK;

@@ -127,8 +127,9 @@ struct Request {
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
+    rk_u32* flag = nullptr;          // pinned host copy of the launch's error flag
     void* dev = nullptr;             // passes | tile_pass | counters | state
-    size_t cap_passes = 0, cap_tiles = 0, cap_lines = 0;
+    size_t cap_passes = 0, cap_tiles = 0, cap_lines = 0, bytes = 0;
     std::vector<rk_batch_pass> passes;
     std::vector<rk_u32> tile_pass;
     std::vector<Request> reqs;       // in flight
@@ -148,7 +149,7 @@ struct rk_server {
     std::unordered_map<uint64_t, std::pair<size_t, size_t>> outputs;   // id -> (offset, bytes)
     std::unordered_set<uint64_t> live;   // ids submitted and not yet released
     std::vector<rk_completion> finished;
-    uint64_t launches = 0, completed = 0;
+    uint64_t launches = 0, completed = 0, failed = 0;
 };
 
 static int destroy_server(rk_server* s) {
@@ -158,6 +159,7 @@ static int destroy_server(rk_server* s) {
         if (sl.done) hipEventDestroy(sl.done);
         if (sl.stream) hipStreamDestroy(sl.stream);
         if (sl.dev) hipFree(sl.dev);
+        if (sl.flag) hipHostFree(sl.flag);
     }
     for (auto& kv : s->templates) {
         if (kv.second->tables) hipFree(kv.second->tables);
@@ -186,6 +188,8 @@ extern "C" int rk_server_create(const rk_server_config* cfg, rk_server** out) {
     for (auto& sl : s->slots) {
         RK_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
         RK_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+        RK_HIP(hipHostMalloc((void**)&sl.flag, 64, hipHostMallocDefault));
+        *sl.flag = 0;
     }
     *out = s.release();
     return RK_OK;
@@ -281,7 +285,7 @@ extern "C" int rk_server_submit(rk_server* s, const rk_request* reqs, int n) {
     return RK_OK;
 }
 
-static void finish_slot(rk_server* s, Slot& sl) {
+static void finish_slot(rk_server* s, Slot& sl, int status) {
     const uint64_t t = now_ns();
     for (Request& q : sl.reqs) {
         if (q.ws_bytes) s->arena.release(q.ws_off, q.ws_bytes);
@@ -292,6 +296,7 @@ static void finish_slot(rk_server* s, Slot& sl) {
         c.t_arrival_ns = q.r.t_arrival_ns;
         c.t_admitted_ns = q.admitted_ns;
         c.t_done_ns = t;
+        c.status = status;
         s->finished.push_back(c);
         s->completed++;
     }
@@ -344,6 +349,7 @@ static int launch_batched(rk_server* s, Slot& sl) {
         sl.cap_lines = std::max<size_t>(lines * 2, 1024);
         const size_t bytes = sl.cap_passes * sizeof(rk_batch_pass) + sl.cap_tiles * 4 + sl.cap_lines * line_bytes + 256;
         RK_HIP(hipMalloc(&sl.dev, bytes));
+        sl.bytes = bytes;
         // on the slot's stream: ordered before the descriptor uploads below
         // (the slot's stream does not wait for the null stream)
         RK_HIP(hipMemsetAsync(sl.dev, 0, bytes, sl.stream));
@@ -358,7 +364,7 @@ static int launch_batched(rk_server* s, Slot& sl) {
     w.state = (rk_u32*)(counters + sl.cap_lines * line_bytes);
     w.n_passes = (rk_u32)sl.passes.size();
     w.total = (rk_u32)total;
-    w.spin_limit = kSpinLimit;
+    w.spin_limit = s->cfg.spin_limit ? s->cfg.spin_limit : kSpinLimit;
     w.poll_sleep_max = rk::batch_poll_sleep_max();
     RK_HIP(hipMemcpyAsync((void*)w.passes, sl.passes.data(), sl.passes.size() * sizeof(rk_batch_pass),
                           hipMemcpyHostToDevice, sl.stream));
@@ -366,6 +372,9 @@ static int launch_batched(rk_server* s, Slot& sl) {
                           sl.stream));
     void* params[] = {&w};
     RK_HIP(hipLaunchKernel(rk::batch_kernel(flavour), dim3(w.total), dim3(RK_BATCH_THREADS), params, 0, sl.stream));
+    // the launch's error flag comes back with it (read when its event completes)
+    *sl.flag = 0;
+    RK_HIP(hipMemcpyAsync(sl.flag, w.state + 2, sizeof(rk_u32), hipMemcpyDeviceToHost, sl.stream));
     RK_HIP(hipEventRecord(sl.done, sl.stream));
     s->launches++;
     return RK_OK;
@@ -390,7 +399,17 @@ extern "C" int rk_server_progress(rk_server* s, rk_completion* done, int max_don
         if (!sl.busy) continue;
         hipError_t e = hipEventQuery(sl.done);
         if (e == hipSuccess) {
-            finish_slot(s, sl);
+            int status = 0;
+            if (!sl.plain && *sl.flag) {
+                // a dependency wait gave up: the launch's results are not valid
+                // and its counters may not have returned to zero; clear the
+                // slot's counters and state before its next launch
+                status = 1;
+                s->failed++;
+                RK_HIP(hipMemsetAsync(sl.dev, 0, sl.bytes, sl.stream));
+                *sl.flag = 0;
+            }
+            finish_slot(s, sl, status);
         } else if (e != hipErrorNotReady) {
             return fail(RK_ERR_HIP, "rk_server_progress: %s", hipGetErrorString(e));
         }
@@ -454,6 +473,7 @@ extern "C" int rk_server_stats(const rk_server* s, rk_server_stats_t* st) {
     st->budget_used = s->arena.used();
     st->budget_bytes = s->arena.size();
     st->templates = (int64_t)s->templates.size();
+    st->failed_launches = (int64_t)s->failed;
     return RK_OK;
 }
 

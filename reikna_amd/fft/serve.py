@@ -54,6 +54,7 @@ class Completion:
     t_arrival_ns: int
     t_admitted_ns: int
     t_done_ns: int
+    status: int = 0   # 1: a dependency wait of its launch gave up, the output is not valid
 
     @property
     def latency_ms(self) -> float:
@@ -69,6 +70,7 @@ class FFTServer:
         mode: str = "batched",
         max_requests: int = 1024,
         plain_min_elements: int = 0,
+        spin_limit: int = 0,
     ):
         if mode not in ("batched", "plain"):
             raise ValueError(f"unknown server mode {mode!r}")
@@ -81,6 +83,7 @@ class FFTServer:
         cfg.max_requests = int(max_requests)
         cfg.mode = _lib.RK_SERVER_BATCHED if mode == "batched" else _lib.RK_SERVER_PLAIN
         cfg.plain_min_elements = int(plain_min_elements)
+        cfg.spin_limit = int(spin_limit)
         h = ctypes.c_void_p()
         _lib.call("rk_server_create", ctypes.byref(cfg), ctypes.byref(h))
         self._h = h.value
@@ -113,7 +116,7 @@ class FFTServer:
         n = ctypes.c_int(0)
         _lib.call("rk_server_progress", self._h, self._done, len(self._done), ctypes.byref(n))
         return [
-            Completion(c.id, int(c.out or 0), c.t_arrival_ns, c.t_admitted_ns, c.t_done_ns)
+            Completion(c.id, int(c.out or 0), c.t_arrival_ns, c.t_admitted_ns, c.t_done_ns, c.status)
             for c in self._done[: n.value]
         ]
 
@@ -275,7 +278,9 @@ def _replay_native(server: FFTServer, specs, rate, t0, arrivals, launches0) -> S
         r.t_arrival_ns = int(arrivals[k])
     done = (_lib.Completion * max(1, n))()
     _lib.call("rk_server_replay", server._h, arr, n, done, 0, 0)
-    comps = [Completion(c.id, 0, c.t_arrival_ns, c.t_admitted_ns, c.t_done_ns) for c in done[:n]]
+    comps = [Completion(c.id, 0, c.t_arrival_ns, c.t_admitted_ns, c.t_done_ns, c.status) for c in done[:n]]
+    if any(c.status for c in comps):
+        raise RuntimeError("request server: a batch-kernel dependency wait gave up; results are not valid")
     t_end = max(c.t_done_ns for c in comps)
     duration = (t_end - t0) / 1e9
     lat = numpy.array([c.latency_ms for c in comps])

@@ -254,3 +254,23 @@ def test_r2c_forward_only(queue):
     out = Array.from_host(queue, numpy.zeros(200, numpy.complex64))
     with pytest.raises(Exception, match="forward only"):
         batch(queue, out, inp, 1)
+
+
+def test_wait_give_up_is_reported_and_cleared(queue, monkeypatch):
+    """A poll limit of 1 makes the second-pass tiles give up their waits (every
+    wait is bounded: the kernel still ends); check() reports it once and
+    clears the plan's flag and counters."""
+    monkeypatch.setenv("RK_BATCH_SPIN_LIMIT", "1")
+    items = packed_items([65536] * 8)
+    batch = BatchFFT(queue.device, items)
+    monkeypatch.delenv("RK_BATCH_SPIN_LIMIT")
+    data = packed_data([65536] * 8, [1] * 8, seed=42)
+    inp = Array.from_host(queue, data)
+    out = Array.from_host(queue, numpy.zeros_like(data))
+    batch(queue, out, inp)
+    with pytest.raises(RuntimeError, match="poll limit"):
+        batch.check()
+    batch.check()   # cleared
+    # a plan with the default limit is unaffected
+    res, _ = run_batch(queue, items, data, data.size)
+    check_items(items, data, res)

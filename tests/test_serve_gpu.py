@@ -156,3 +156,32 @@ def test_ids_in_use_and_busy_replay_refused(queue):
     res = run_stream(server, specs, rate=1000.0)
     assert res.completed == 2
     server.close()
+
+
+def test_failed_waits_are_reported(queue):
+    """A launch whose dependency waits give up (forced with a poll limit of 1:
+    every wait is bounded, so every kernel still ends) is reported on its
+    completions and in the stats, and the server keeps serving."""
+    specs = make_specs(queue, [((65536,), 16), ((65536,), 16)], seed=40)
+    server = FFTServer(queue.device, budget_bytes=256 * 2**20, slots=1, spin_limit=1)
+    server.submit([(i, s, 0) for i, s in enumerate(specs)])
+    comps = []
+    while len(comps) < len(specs):
+        comps += server.progress()
+    assert any(c.status == 1 for c in comps)
+    assert server.stats["failed_launches"] >= 1
+    for c in comps:
+        server.release(c.id)
+    # a single-pass request needs no waits: served correctly after the failure
+    small = make_specs(queue, [((1024,), 4)], seed=41)
+    server.submit([(10, small[0], 0)])
+    done = []
+    while not done:
+        done = server.progress()
+    assert done[0].status == 0
+    check_output(server, queue, done[0], small[0], {})
+    server.release(10)
+    # the native replay raises on a failed launch
+    with pytest.raises(RuntimeError, match="gave up"):
+        run_stream(server, specs, rate=1000.0)
+    server.close()
