@@ -133,7 +133,22 @@ int tile_lds_bytes_host(int L, int C) {
     return C * rkfft::lds_pitch_for(L, C, threads_per_instance(L)) * (rkfft::lds_split(L, C) ? 4 : 8);
 }
 
+// Tuning overrides of the tile widths for experiments (kernels outside the
+// catalog come from the run-time compiler): RK_ROW_TILE / RK_COL_TILE force
+// the instances per row / column tile when the tile stays within 1024
+// threads and 159 KB of LDS.
+static int forced_tile(const char* name, int L) {
+    const char* e = getenv(name);
+    const int c = e && *e ? atoi(e) : 0;
+    if (c <= 0) return 0;
+    if ((int64_t)c * threads_per_instance(L) > 1024 || (int64_t)c * lds_pitch_host(L) * 8 > 159 * 1024) return 0;
+    return c;
+}
+
 int tile_c_row(int L) {
+    static const bool any = getenv("RK_ROW_TILE") != nullptr;
+    if (any)
+        if (int f = forced_tile("RK_ROW_TILE", L)) return f;
     int T = threads_per_instance(L);
     int C = std::max(1, 256 / T);
     while (C > 1 && (int64_t)C * lds_pitch_host(L) * 8 > 64 * 1024) C /= 2;
@@ -141,6 +156,9 @@ int tile_c_row(int L) {
 }
 
 int tile_c_col(int L) {
+    static const bool any = getenv("RK_COL_TILE") != nullptr;
+    if (any)
+        if (int f = forced_tile("RK_COL_TILE", L)) return f;
     int T = threads_per_instance(L);
     int C = std::max(16, (tile_c_row(L) / 16) * 16);
     while (C > 1 && ((int64_t)C * T > 1024 || (int64_t)C * lds_pitch_host(L) * 8 > 159 * 1024)) C /= 2;
