@@ -90,6 +90,15 @@ def cpu_baseline(shape, axes, data, budget_s: float = 12.0) -> dict:
         times.append(time.perf_counter() - t0)
     t = min(times)
     flops = 5.0 * sample.size * sum(math.log2(shape[a]) for a in axes)
+    # numpy.fft (pocketfft, one thread) on a smaller slice of the same sample
+    one = sample if 0 in axes else sample[: max(1, sample_rows // 4)]
+    one_flops = 5.0 * one.size * sum(math.log2(shape[a]) for a in axes)
+    one_times = []
+    t_end = time.perf_counter() + budget_s / 2
+    while len(one_times) < 5 and (time.perf_counter() < t_end or len(one_times) < 1):
+        t0 = time.perf_counter()
+        numpy.fft.fftn(one, axes=axes)
+        one_times.append(time.perf_counter() - t0)
     cpu_model = ""
     try:
         for line in open("/proc/cpuinfo"):
@@ -105,6 +114,10 @@ def cpu_baseline(shape, axes, data, budget_s: float = 12.0) -> dict:
         "kind": "port",
         "sample": f"{lib}, {sample.shape} complex64 over axes {tuple(axes)} ({sample_rows} of {shape[0]} rows), "
         f"min of {len(times)} runs; host {cpu_model}, os.cpu_count()={os.cpu_count()}",
+        "numpy_1_thread": {
+            "value": round(one_flops / min(one_times) / 1e9, 2),
+            "sample": f"numpy.fft {numpy.__version__} (pocketfft, one thread), {one.shape}, min of {len(one_times)} runs",
+        },
     }
 
 
@@ -168,6 +181,17 @@ def main(argv=None) -> int:
     wall = time.perf_counter() - t0
     ms_total = e0.elapsed_ms(e1)
     ms_per_step = ms_total / args.steps
+
+    # individually timed steps: min and median
+    step_ms = []
+    for _ in range(max(20, args.steps)):
+        s0, s1 = Event(), Event()
+        s0.record(queue)
+        fftc(queue, arr_out, arr_in, 0)
+        s1.record(queue)
+        s1.synchronize()
+        step_ms.append(s0.elapsed_ms(s1))
+    step_ms.sort()
 
     # per-pass kernel durations, HIP events on the stream the kernels run on
     native.set_timing(True)
@@ -244,6 +268,8 @@ def main(argv=None) -> int:
             "d2d_copy_GBs": round(copy_gbs, 1),
             "transform_frac_of_copy": round(transform_gbs / copy_gbs, 4),
         },
+        "step_ms_min": round(step_ms[0], 5),
+        "step_ms_median": round(step_ms[len(step_ms) // 2], 5),
         "rel_l2": err,
         "rel_l2_bound": tol,
         "wall_s": round(wall, 4),
