@@ -133,3 +133,22 @@ def test_pure_parallel_from_transformation_compiles(vdev):
     c = pp.compile(vdev)
     assert len(c.kernel_sources) == 1
     assert "rk_skip" in c.kernel_sources[0]
+
+
+def test_plan_streaming_accesses(vdev):
+    # passes whose input and output exceed the 256 MB Infinity Cache stream
+    # (nontemporal kernels); the in-place passes of a 128 MiB 3-D transform
+    # and small transforms keep plain accesses
+    assert all(p.stream for p in passes((64, 2**20), (1,), dev=vdev))
+    assert all(p.stream for p in passes((4096, 12288), (1,), dev=vdev))
+    assert not any(p.stream for p in passes((256, 256, 256), (0, 1, 2), dev=vdev))
+    assert not any(p.stream for p in passes((1024, 1024), (1,), dev=vdev))
+    # Bluestein passes have no streaming twins
+    assert not any(p.stream for p in passes((4096, 65521), (1,), dev=vdev) if p.variant & 2)
+
+
+def test_plan_xcd_groups_for_narrow_column_tiles(vdev):
+    # a 4096-point strided axis: column tiles of 4 instances (the pass is planned;
+    # its XCD grouping is a launch detail checked on the GPU)
+    ps = passes((4096, 40), (0,), dev=vdev)
+    assert [(p.L, p.C, p.load_mode) for p in ps] == [(4096, 4, "direct")]
