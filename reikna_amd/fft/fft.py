@@ -67,6 +67,9 @@ class FFT(Computation):
         arr = _meta(arr_t)
         if not dtypes.is_complex(arr.dtype):
             raise ValueError("FFT computation requires array of a complex dtype")
+        if arr.dtype != numpy.complex64:
+            # the kernels compute in complex64 (FP32 butterflies, FP64-derived tables)
+            raise NotImplementedError(f"FFT of {arr.dtype}: this implementation computes complex64 transforms")
         Computation.__init__(
             self,
             [

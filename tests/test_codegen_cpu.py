@@ -152,3 +152,11 @@ def test_plan_xcd_groups_for_narrow_column_tiles(vdev):
     # its XCD grouping is a launch detail checked on the GPU)
     ps = passes((4096, 40), (0,), dev=vdev)
     assert [(p.L, p.C, p.load_mode) for p in ps] == [(4096, 4, "direct")]
+
+
+def test_complex128_refused():
+    # the kernels compute complex64; a complex128 array is refused, not misread
+    with pytest.raises(NotImplementedError, match="complex64"):
+        FFT(ArrayMetadata((4, 64), numpy.complex128))
+    with pytest.raises(ValueError, match="complex dtype"):
+        FFT(ArrayMetadata((4, 64), numpy.float64))
