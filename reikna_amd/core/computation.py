@@ -347,6 +347,30 @@ def pack_arguments(values: Sequence[tuple[Annotation, Any]]) -> bytes:
     return bytes(buf)
 
 
+def _check_array_argument(name: str, val: Any, meta: ArrayMetadata) -> None:
+    """The kernels address an array argument with the layout of its parameter
+    (compiled in): an array of another shape, type or layout, or one whose
+    buffer is smaller than that layout spans, is refused instead of being
+    read or written out of place."""
+    if isinstance(val, Array):
+        arr = val
+    elif hasattr(val, "data_ptr") and hasattr(val, "is_cuda"):   # torch tensor
+        arr = Array.from_torch(val)
+    else:
+        raise TypeError(f"{name}: expected a device Array, got {type(val)}")
+    have = arr.metadata
+    if have.shape != meta.shape or have.dtype != meta.dtype:
+        raise ValueError(f"{name}: array of shape {have.shape} and type {have.dtype}, "
+                         f"the computation was compiled for {meta.shape} {meta.dtype}")
+    moved = any(n > 1 and a != b for n, a, b in zip(meta.shape, have.strides, meta.strides))
+    if moved or have.first_element_offset != meta.first_element_offset:
+        raise ValueError(f"{name}: array strides {have.strides} / offset {have.first_element_offset} differ "
+                         f"from the compiled {meta.strides} / {meta.first_element_offset}")
+    size = getattr(arr.buffer, "size", None)
+    if size is not None and size < meta.buffer_size:
+        raise ValueError(f"{name}: buffer of {size} bytes, the compiled layout spans {meta.buffer_size}")
+
+
 def _array_ptr(val: Any) -> int:
     if isinstance(val, Array):
         return val.ptr
@@ -619,6 +643,10 @@ class ComputationCallable:
 
     def __call__(self, queue: Queue, *args: Any, **kwds: Any) -> list[Any]:
         bound = self.signature.bind_with_defaults(args, kwds, cast=True)
+        for name, val in bound.arguments.items():
+            ann = self.signature.parameters[name].annotation
+            if ann.is_array():
+                _check_array_argument(name, val, ann.array_metadata)
         values = dict(self._internal_args)
         values.update(bound.arguments)
         for step in self._steps:

@@ -160,3 +160,22 @@ def test_complex128_refused():
         FFT(ArrayMetadata((4, 64), numpy.complex128))
     with pytest.raises(ValueError, match="complex dtype"):
         FFT(ArrayMetadata((4, 64), numpy.float64))
+
+
+def test_array_arguments_checked_against_the_compiled_layout():
+    from reikna_amd.core.computation import _check_array_argument
+    from reikna_amd.runtime import Array, _ForeignBuffer
+
+    meta = ArrayMetadata((512,), numpy.complex64, strides=(16,), buffer_size=8192)
+    dev = VirtualDevice()
+
+    def arr(m, size):
+        return Array(dev, m, _ForeignBuffer(dev, 0, size, None))
+
+    _check_array_argument("a", arr(meta, 8192), meta)
+    with pytest.raises(ValueError, match="strides"):
+        _check_array_argument("a", arr(ArrayMetadata((512,), numpy.complex64), 4096), meta)
+    with pytest.raises(ValueError, match="bytes"):
+        _check_array_argument("a", arr(meta, 4096), meta)
+    with pytest.raises(ValueError, match="shape"):
+        _check_array_argument("a", arr(ArrayMetadata((256,), numpy.complex64), 8192), meta)
