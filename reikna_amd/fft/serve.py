@@ -61,6 +61,14 @@ class Completion:
         return (self.t_done_ns - self.t_arrival_ns) / 1e6
 
 
+def _check_spec(rid: int, spec: RequestSpec) -> None:
+    """The kernels read count * prod(shape) complex64 values from the input."""
+    need = int(spec.count) * math.prod(spec.shape)
+    if spec.input.dtype != numpy.complex64 or spec.input.size < need:
+        raise ValueError(f"request {rid}: input must be complex64 with at least {need} elements "
+                         f"(got {spec.input.dtype} x {spec.input.size})")
+
+
 class FFTServer:
     def __init__(
         self,
@@ -101,6 +109,7 @@ class FFTServer:
             return
         arr = (_lib.Request * len(reqs))()
         for k, (rid, spec, t_arr) in enumerate(reqs):
+            _check_spec(rid, spec)
             r = arr[k]
             r.id = int(rid)
             r.ndim = len(spec.shape)
@@ -267,6 +276,7 @@ def _replay_native(server: FFTServer, specs, rate, t0, arrivals, launches0) -> S
     n = len(specs)
     arr = (_lib.Request * max(1, n))()
     for k, spec in enumerate(specs):
+        _check_spec(k, spec)
         r = arr[k]
         r.id = k
         r.ndim = len(spec.shape)
