@@ -241,3 +241,37 @@ def test_long_strided_axis(queue, shape):
         res, _ = run_fft(queue, data, (0,), inverse=inverse, inplace=False)
         ref = oracle.ifftn(data, (0,)) if inverse else oracle.fftn(data, (0,))
         assert oracle.rel_l2(res, ref) <= oracle.tolerance(shape, (0,))
+
+
+def test_graph_capture_of_plans(queue):
+    """A multi-pass plan (workspace ping-pong) and a plan with fused
+    transformations recorded once as a HIP graph and replayed several times,
+    with eager work and syncs in between, each replay equal to the eager run."""
+    data = oracle.test_array((8, 2**16), seed=31)
+    arr = Array.from_host(queue, data)
+    out = Array.empty_like(queue.device, arr)
+    fft = FFT(arr, axes=(1,)).compile(queue.device)
+    fft(queue, out, arr, 0)
+    eager = out.get(queue)
+    assert oracle.rel_l2(eager, oracle.fftn(data, (1,))) <= oracle.tolerance((8, 2**16), (1,))
+
+    arr2 = Array.from_host(queue, data)
+    fused = FFT(arr2, axes=(1,))
+    scale = div_const(arr2, 4.0)
+    fused.parameter.output.connect(scale, scale.input, scaled=scale.output)
+    fusedc = fused.compile(queue.device)
+    out2 = Array.empty_like(queue.device, arr2)
+    fusedc(queue, out2, arr2, 0)
+    eager2 = out2.get(queue)
+
+    with queue.capture() as cap:
+        fft(queue, out, arr, 0)
+        fusedc(queue, out2, arr2, 0)
+    for k in range(3):
+        out.set(queue, numpy.zeros_like(data))
+        out2.set(queue, numpy.zeros_like(data))
+        cap.graph.launch(queue)
+        assert numpy.array_equal(out.get(queue), eager), k
+        assert numpy.array_equal(out2.get(queue), eager2), k
+        fft(queue, out, arr, 1)          # eager work between replays
+        queue.synchronize()
