@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
     # name: (shape, axes, distribution, seed, description)
     "c2b": ((64, 2**20), (1,), "normal", 3, "c64 C2C FFT 64 x 2^20 over axis 1"),
-    "c1": ((1024, 1024), (1,), "uniform", 1, "c64 C2C FFT 1024 x 1024 over axis 1"),
+    "c1": ((1024, 1024), (1,), "uniform", 1, "c64 C2C FFT 1024 x 1024 over axis 1, forward + inverse"),
     "c2a": ((256, 256, 256), (0, 1, 2), "normal", 2, "c64 C2C FFT 256^3 over axes (0, 1, 2)"),
     "c3-1000": ((4096, 1000), (1,), "uniform", 4, "c64 C2C FFT 4096 x 1000 over axis 1"),
     "c3-997": ((4096, 997), (1,), "uniform", 4, "c64 C2C FFT 4096 x 997 (prime, Bluestein) over axis 1"),
@@ -51,6 +51,7 @@ WORKLOADS = {
     "spectrogram": ((2**24,), None, "normal", 15, "f32 Spectrogram of 2^24 samples, 256-point frames, overlap 128"),
 }
 COMPUTATIONS = ("transpose", "rfft", "irfft", "fftshift", "spectrogram")
+FWD_INV = ("c1",)   # a step is the forward and the inverse transform
 
 # HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE
 # and WRITE_SIZE runs of this benchmark (profiles/fft_pass_c2b_64x2p20.md):
@@ -64,21 +65,25 @@ MEASURED_TRAFFIC = {
 }
 
 
-def cpu_baseline(shape, axes, data, budget_s: float = 12.0) -> dict:
+def cpu_baseline(shape, axes, data, budget_s: float = 12.0, fwd_inv: bool = False) -> dict:
     """The oracle library on this host's cores: scipy.fft (pocketfft) with one
-    worker per core over a sample of the rows of the workload."""
+    worker per core over a sample of the rows of the workload (forward and
+    inverse when the workload's step is both)."""
     import numpy
 
     try:
         import scipy.fft as sfft
 
         workers = len(os.sched_getaffinity(0))
-        fn = lambda x: sfft.fftn(x, axes=axes, workers=workers)  # noqa: E731
+        fwd = lambda x: sfft.fftn(x, axes=axes, workers=workers)  # noqa: E731
+        inv = lambda x: sfft.ifftn(x, axes=axes, workers=workers)  # noqa: E731
         lib = f"scipy.fft {__import__('scipy').__version__} (pocketfft)"
     except Exception:   # pragma: no cover
         workers = 1
-        fn = lambda x: numpy.fft.fftn(x, axes=axes)  # noqa: E731
+        fwd = lambda x: numpy.fft.fftn(x, axes=axes)  # noqa: E731
+        inv = lambda x: numpy.fft.ifftn(x, axes=axes)  # noqa: E731
         lib = f"numpy.fft {numpy.__version__} (pocketfft)"
+    fn = (lambda x: inv(fwd(x))) if fwd_inv else fwd  # noqa: E731
     rows = shape[0] if 0 not in axes else shape[0]
     sample_rows = rows if 0 in axes else max(1, min(rows, (8 * 2**20) // max(1, math.prod(shape[1:]))))
     sample = data if 0 in axes else data[:sample_rows]
@@ -89,15 +94,17 @@ def cpu_baseline(shape, axes, data, budget_s: float = 12.0) -> dict:
         fn(sample)
         times.append(time.perf_counter() - t0)
     t = min(times)
-    flops = 5.0 * sample.size * sum(math.log2(shape[a]) for a in axes)
+    flops = (2.0 if fwd_inv else 1.0) * 5.0 * sample.size * sum(math.log2(shape[a]) for a in axes)
     # numpy.fft (pocketfft, one thread) on a smaller slice of the same sample
     one = sample if 0 in axes else sample[: max(1, sample_rows // 4)]
-    one_flops = 5.0 * one.size * sum(math.log2(shape[a]) for a in axes)
+    one_flops = (2.0 if fwd_inv else 1.0) * 5.0 * one.size * sum(math.log2(shape[a]) for a in axes)
     one_times = []
     t_end = time.perf_counter() + budget_s / 2
     while len(one_times) < 5 and (time.perf_counter() < t_end or len(one_times) < 1):
         t0 = time.perf_counter()
-        numpy.fft.fftn(one, axes=axes)
+        y = numpy.fft.fftn(one, axes=axes)
+        if fwd_inv:
+            numpy.fft.ifftn(y, axes=axes)
         one_times.append(time.perf_counter() - t0)
     cpu_model = ""
     try:
@@ -168,14 +175,22 @@ def main(argv=None) -> int:
     passes = native.passes
     print(f"[bench] passes: {[(p.L, p.C, p.load_mode, p.store_mode) for p in passes]}", file=sys.stderr, flush=True)
 
-    for _ in range(args.warmup):
+    fwd_inv = args.workload in FWD_INV
+    arr_back = Array.empty_like(dev, arr_in) if fwd_inv else None
+
+    def step():
         fftc(queue, arr_out, arr_in, 0)
+        if fwd_inv:
+            fftc(queue, arr_back, arr_out, 1)
+
+    for _ in range(args.warmup):
+        step()
     queue.synchronize()
     e0, e1 = Event(), Event()
     e0.record(queue)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        fftc(queue, arr_out, arr_in, 0)
+        step()
     e1.record(queue)
     queue.synchronize()
     wall = time.perf_counter() - t0
@@ -187,7 +202,7 @@ def main(argv=None) -> int:
     for _ in range(max(20, args.steps)):
         s0, s1 = Event(), Event()
         s0.record(queue)
-        fftc(queue, arr_out, arr_in, 0)
+        step()
         s1.record(queue)
         s1.synchronize()
         step_ms.append(s0.elapsed_ms(s1))
@@ -216,7 +231,7 @@ def main(argv=None) -> int:
     c1.record(queue)
     queue.synchronize()
     copy_gbs = 2.0 * arr_in.nbytes * ncopy / (c0.elapsed_ms(c1) / 1e3) / 1e9
-    fftc(queue, arr_out, arr_in, 0)
+    step()
 
     # correctness spot check against the FP64 oracle
     res = arr_out.get(queue)
@@ -226,13 +241,16 @@ def main(argv=None) -> int:
         rows = [0, shape[0] - 1]
         err = max(oracle.rel_l2(res[r], oracle.fftn(data[r], [a - 1 for a in axes])) for r in rows)
     tol = oracle.tolerance(shape, axes)
+    if fwd_inv:   # the inverse of the forward result gives the input back
+        err = max(err, oracle.rel_l2(arr_back.get(queue), data) / 2)
 
     elems = math.prod(shape)
-    flops = oracle.fft_flops(shape, axes)
+    directions = 2 if fwd_inv else 1
+    flops = directions * oracle.fft_flops(shape, axes)
     gflops = flops / (ms_per_step / 1e3) / 1e9
     alg_bytes = 16.0 * elems
     achieved = alg_bytes / (pass_ms[dom] / 1e3) / 1e9
-    transform_gbs = alg_bytes / (ms_per_step / 1e3) / 1e9
+    transform_gbs = directions * alg_bytes / (ms_per_step / 1e3) / 1e9
     out = {
         "metric": METRIC,
         "value": round(gflops, 1),
@@ -251,6 +269,7 @@ def main(argv=None) -> int:
             "shape": list(shape),
             "axes": list(axes),
             "distribution": f"{dist}, seed {seed}",
+            "directions": "forward + inverse" if fwd_inv else "forward",
             "passes": [{"L": p.L, "C": p.C, "load": p.load_mode, "store": p.store_mode, "stream": p.stream}
                        for p in passes],
         },
@@ -276,7 +295,7 @@ def main(argv=None) -> int:
     }
     if not args.no_cpu:
         print("[bench] CPU baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(shape, axes, data)
+        out["cpu_baseline"] = cpu_baseline(shape, axes, data, fwd_inv=fwd_inv)
     print(json.dumps(out), flush=True)
     return 0 if err <= tol else 1
 
