@@ -275,3 +275,17 @@ def test_graph_capture_of_plans(queue):
         assert numpy.array_equal(out2.get(queue), eager2), k
         fft(queue, out, arr, 1)          # eager work between replays
         queue.synchronize()
+
+
+@pytest.mark.parametrize("case", ["n8", "n60", "n997", "n1000", "n1024", "n12288", "d2_17x200", "d3_8x12x10"])
+def test_golden_fixtures(queue, case):
+    """The HIP path against the committed FP64 fixtures (tests/golden/, written
+    by make_golden.py from numpy.fft), forward and inverse."""
+    from pathlib import Path
+
+    g = numpy.load(Path(__file__).parent / "golden" / "fft_golden.npz")
+    data, axes = g[case + "_in"], tuple(int(a) for a in g[case + "_axes"])
+    tol = oracle.tolerance(data.shape, axes)
+    for inverse, key in ((False, "_fwd"), (True, "_inv")):
+        res, _ = run_fft(queue, data, axes, inverse=inverse, inplace=False)
+        assert oracle.rel_l2(res, g[case + key]) <= tol, (case, inverse)
