@@ -50,3 +50,26 @@ def test_errors_without_device_are_reported():
     # a null descriptor is rejected with a message, not a crash
     assert lib.rk_fft_plan_create(None, None) == _lib.RK_ERR_INVALID
     assert "null" in _lib.last_error()
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """Without the HIP library every call raises: there is no CPU path."""
+    import subprocess
+    import sys
+
+    code = (
+        "import numpy\n"
+        "from reikna_amd import ArrayMetadata\n"
+        "from reikna_amd.fft import FFT\n"
+        "from reikna_amd.runtime import VirtualDevice\n"
+        "try:\n"
+        "    FFT(ArrayMetadata((4, 64), numpy.complex64), axes=(1,)).compile(VirtualDevice())\n"
+        "except Exception as e:\n"
+        "    print('raised:', type(e).__name__, e)\n"
+        "else:\n"
+        "    print('no error')\n"
+    )
+    env = dict(__import__("os").environ, REIKNA_AMD_LIB=str(tmp_path / "absent.so"))
+    root = __import__("pathlib").Path(__file__).resolve().parent.parent
+    out = subprocess.run([sys.executable, "-c", code], cwd=root, env=env, capture_output=True, text=True, timeout=120)
+    assert "raised: RkError" in out.stdout and "not built" in out.stdout, out.stdout + out.stderr
