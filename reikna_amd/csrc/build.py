@@ -133,6 +133,7 @@ VAR_TWIDDLE_CONJ_CHIRP = 1 | (2 << 2)  # last pass of the second Bluestein chain
 VAR_BLUE = 2 | (2 << 2) | (1 << 4)   # single-kernel Bluestein
 VAR_STREAM = 1 << 7                  # catalogue kernel with nontemporal loads/stores (StreamIO)
 STREAM_MAX = 4096                    # streaming kernels for plain/twiddle passes up to this length
+BLUE_SINGLE_MAX = 8192               # longest single-kernel Bluestein convolution (rk_fft_plan.cpp)
 
 
 def catalog_entries() -> list[tuple[int, int, int]]:
@@ -140,7 +141,7 @@ def catalog_entries() -> list[tuple[int, int, int]]:
     for length in catalog_lengths():
         ents.add((length, tile_c_row(length), VAR_PLAIN))
         ents.add((length, tile_c_col(length), VAR_PLAIN))
-        if length >= 8:
+        if 8 <= length <= BLUE_SINGLE_MAX:
             ents.add((length, tile_c_row(length), VAR_BLUE))
         if 16 <= length <= 4096:
             for v in (VAR_TWIDDLE, VAR_CHIRP, VAR_TWIDDLE_KMUL, VAR_TWIDDLE_CONJ_CHIRP):

@@ -37,7 +37,11 @@ namespace rk {
 // of short DFTs: 4096 x 12288 took 413 us in one pass and 345 us in two
 // (96 x 128).  A single-kernel Bluestein convolution reads and writes its
 // data once instead of running two chains through a workspace of twice the
-// length, so it keeps the longer limit.
+// length, so it keeps a longer limit, up to the point where its tile of 640+
+// threads runs out of registers: 4096 rows of 4093 points (M = 8192) took
+// 350 us in one kernel and 453 us as two chains, 4096 x 5003 (M = 10240)
+// 1085 us in one kernel and 634 us as two chains, 4096 x 7919 (M = 16384)
+// 1177 and 891 us.
 static int64_t env_single_max() {
     static const int64_t v = [] {
         const char* e = getenv("RK_SINGLE_MAX");
@@ -46,7 +50,7 @@ static int64_t env_single_max() {
     return v;
 }
 static int64_t single_max() { return env_single_max() ? env_single_max() : 4096; }
-static int64_t blue_single_max() { return env_single_max() ? env_single_max() : 16384; }
+static int64_t blue_single_max() { return env_single_max() ? env_single_max() : 8192; }
 #define kSingleMax (single_max())
 
 // Longest DFT of one pass in a multi-pass chain.  Column-style passes need

@@ -51,11 +51,12 @@ def test_plan_three_axes(vdev):
     assert all(p.dst == "out" for p in ps)   # single-pass axes run in place on `out`
 
 
-@pytest.mark.parametrize("n,expect", [(997, 2048), (1009, 2048), (5003, 10240), (12289, 32768), (65521, 131072)])
+@pytest.mark.parametrize("n,expect", [(997, 2048), (1009, 2048), (4093, 8192), (5003, 10240), (12289, 32768),
+                                      (65521, 131072)])
 def test_plan_bluestein_length(vdev, n, expect):
     ps = passes((4, n), (1,), dev=vdev)
     assert all(p.axis_len == expect for p in ps)
-    if expect <= 16384:
+    if expect <= 8192:
         assert len(ps) == 1 and ps[0].kind == 1   # single-kernel Bluestein
     else:
         # forward chain over M, then a second forward chain of the same passes
