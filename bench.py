@@ -131,8 +131,10 @@ def cpu_baseline(shape, axes, data, budget_s: float = 12.0, fwd_inv: bool = Fals
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    # a few dozen untimed steps: the first hundred milliseconds of work on a
+    # fresh process run ~6 % slower than the steady state that follows
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="c2b", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--autotune", action="store_true", help="c4: pick batch planner knobs by measurement")

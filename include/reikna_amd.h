@@ -208,6 +208,7 @@ typedef struct {
     int variant;          /* compile-time kernel variant (pre-op | post-op << 2 | bluestein << 4) */
     int64_t in_dims[3];   /* folded (O, axis length read, I) of the pass input */
     int stream;           /* 1 if the kernel loads and stores with the nontemporal hint */
+    int fixed_modes;      /* 1 if the kernel has load_mode/store_mode compiled in */
 } rk_fft_pass_info;
 
 int rk_fft_plan_num_passes(const rk_fft_plan* plan, int* n);

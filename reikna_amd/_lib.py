@@ -152,6 +152,7 @@ class FftPassInfo(ctypes.Structure):
         ("variant", ctypes.c_int),
         ("in_dims", ctypes.c_int64 * 3),
         ("stream", ctypes.c_int),
+        ("fixed_modes", ctypes.c_int),
     ]
 
 

@@ -134,6 +134,9 @@ VAR_BLUE = 2 | (2 << 2) | (1 << 4)   # single-kernel Bluestein
 VAR_STREAM = 1 << 7                  # catalogue kernel with nontemporal loads/stores (StreamIO)
 STREAM_MAX = 4096                    # streaming kernels for plain/twiddle passes up to this length
 BLUE_SINGLE_MAX = 8192               # longest single-kernel Bluestein convolution (rk_fft_plan.cpp)
+VAR_MODE_DD = 1 << 8                 # load/store modes DIRECT/DIRECT compiled in (RK_VAR_MODE_DD)
+VAR_MODE_DF = 1 << 9                 # load/store modes DIRECT/FLAT compiled in (RK_VAR_MODE_DF)
+FACTOR_MAX = 256                     # longest pass of a multi-pass chain (rk_fft_plan.cpp factor_max)
 
 
 def catalog_entries() -> list[tuple[int, int, int]]:
@@ -150,6 +153,21 @@ def catalog_entries() -> list[tuple[int, int, int]]:
     for length, c, v in list(ents):
         if length <= STREAM_MAX and v in (VAR_PLAIN, VAR_TWIDDLE):
             ents.add((length, c, v | VAR_STREAM))
+    # the column passes of large multi-pass transforms with their modes
+    # compiled in: the first pass loads columns and stores the tile flat (or
+    # in columns), the later passes load and store columns.  Only lengths
+    # with a factor 3, 5 or 7: their tiles are register-bound (146 VGPRs with
+    # every mode, 70-83 with two) and run 2-5 % faster with the modes fixed;
+    # power-of-two tiles are LDS-bound at four waves per SIMD either way and
+    # measured 1-2 % slower with them.
+    for length, c, v in list(ents):
+        pow2 = length & (length - 1) == 0
+        if not pow2 and length <= FACTOR_MAX and c == tile_c_col(length) and v & VAR_STREAM:
+            if v == VAR_PLAIN | VAR_STREAM:
+                ents.add((length, c, v | VAR_MODE_DF))
+                ents.add((length, c, v | VAR_MODE_DD))
+            elif v == VAR_TWIDDLE | VAR_STREAM:
+                ents.add((length, c, v | VAR_MODE_DD))
     return sorted(ents)
 
 
@@ -219,7 +237,7 @@ def generate_batch_cases(out_dir: Path) -> list[Path]:
 
 
 def variant_cost(v: int) -> float:
-    v &= ~VAR_STREAM
+    v &= ~(VAR_STREAM | VAR_MODE_DD | VAR_MODE_DF)
     return 2.0 if v & (1 << 4) else 1.3 if v else 1.0
 
 
@@ -242,10 +260,13 @@ def generate_catalog(out_dir: Path) -> list[Path]:
             "template <int L, int C, int V>",
             "__global__ void __launch_bounds__((rkfft::pass_threads<L, C>()))",
             "rk_fft_pass(rk_pass_args a) {",
+            "    constexpr int VB = V & (RK_VAR_STREAM - 1);",
+            "    constexpr int M = (V & RK_VAR_MODE_DD) ? RK_MODES(RK_MODE_DIRECT, RK_MODE_DIRECT)",
+            "                    : (V & RK_VAR_MODE_DF) ? RK_MODES(RK_MODE_DIRECT, RK_MODE_FLAT) : 0;",
             "    if constexpr ((V & RK_VAR_STREAM) != 0)",
-            "        rkfft::fft_pass_body<L, C, (V & ~RK_VAR_STREAM)>(a, rkfft::StreamIO{});",
+            "        rkfft::fft_pass_body<L, C, VB, rkfft::StreamIO, M>(a, rkfft::StreamIO{});",
             "    else",
-            "        rkfft::fft_pass_body<L, C, V>(a, rkfft::PlainIO{});",
+            "        rkfft::fft_pass_body<L, C, VB, rkfft::PlainIO, M>(a, rkfft::PlainIO{});",
             "}",
             "static const rk::KernelEntry entries[] = {",
         ]

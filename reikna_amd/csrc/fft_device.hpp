@@ -434,6 +434,16 @@ struct CoherentIO {
 // Bit of a catalogue kernel's variant number (not of Variant<V>): the kernel
 // uses StreamIO instead of PlainIO.
 #define RK_VAR_STREAM 128
+// Bits of a catalogue kernel's variant number: the kernel has the pass's load
+// and store modes compiled in (DIRECT/DIRECT, DIRECT/FLAT) instead of reading
+// them from the pass arguments.  The run-time switch keeps the code of every
+// mode alive in one kernel: a 96-point tile holds 146 VGPRs with it and 75-82
+// with its modes fixed.
+#define RK_VAR_MODE_DD 256
+#define RK_VAR_MODE_DF 512
+// MODES template argument of fft_pass_tile: 0 reads the modes from the pass
+// arguments, RK_MODES(load, store) compiles them in.
+#define RK_MODES(load, store) (16 | (int)(load) | ((int)(store) << 2))
 template <int V> struct Variant {
     static constexpr int PRE = V & 3;
     static constexpr int POST = (V >> 2) & 3;
@@ -626,7 +636,7 @@ template <int L, int C> __host__ __device__ constexpr int tile_lds_bytes() {
 // (a, tile, tid) alone, so a caller with more threads than C*T may give the
 // surplus threads tid = C*T - 1: they repeat that thread's work, writing the
 // same values to the same addresses, and take part in every barrier.
-template <int L, int C, int V, class IO>
+template <int L, int C, int V, class IO, int MODES = 0>
 __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& io, float2* lds, rk_u64 tile, int tid) {
     using P = Plan<L>;
     constexpr int T = P::T;
@@ -641,6 +651,8 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
     constexpr bool TWID = Variant<V>::PRE == RK_VAR_PRE_TWIDDLE;
     constexpr bool SPLIT = tile_split<L, C>();
     const rk_u64 inst0 = tile * (rk_u64)C;
+    const rk_u32 load_mode = MODES ? (rk_u32)(MODES & 3) : a.load_mode;
+    const rk_u32 store_mode = MODES ? (rk_u32)((MODES >> 2) & 3) : a.store_mode;
 
     // compute mapping (see fft_args.h): lanes along an instance or across instances
     const bool colmap = a.cmap != 0;
@@ -657,7 +669,7 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
     // ---- stage-0 inputs: element p = t + r*T of instance c ----
     const Inst ic = decompose<L>(a, inst0 + c);
     auto own_pos = [&](int i) { return cbase + lds_pad(stage_in_pos<L, 0>(t, i)); };
-    if (a.load_mode == RK_MODE_DIRECT) {
+    if (load_mode == RK_MODE_DIRECT) {
 #pragma unroll
         for (int r = 0; r < R0; ++r) v[r] = load_elem<L, V, IO>(a, io, ic, t + r * T);
         if constexpr (TWID) {
@@ -666,7 +678,7 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
 #pragma unroll
             for (int r = 0; r < R0; ++r) v[r] = c_mul(v[r], tw.next());
         }
-    } else if (a.load_mode == RK_MODE_FLAT) {
+    } else if (load_mode == RK_MODE_FLAT) {
         // instance-linear block of C*L elements, no pre-ops
         const rk_i64 base = a.in_off + (rk_i64)inst0 * L;
 #pragma unroll
@@ -724,11 +736,11 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
     }
 
     // ---- store: output element k = stage_out_pos of the last stage ----
-    if (a.store_mode == RK_MODE_DIRECT) {
+    if (store_mode == RK_MODE_DIRECT) {
 #pragma unroll
         for (int i = 0; i < QL * RL; ++i)
             if (stage_valid<L, SL>(t, i / RL)) store_elem<L, V, IO>(a, io, ic, stage_out_pos<L, SL>(t, i), v[i]);
-    } else if (a.store_mode == RK_MODE_FLAT) {
+    } else if (store_mode == RK_MODE_FLAT) {
         // no post-ops beyond sign and scale
         lds_exchange<SPLIT, QL * RL, R0>(lds, v, g, last_out, [&](int it) {
             const int e = tid + it * NT;
@@ -761,13 +773,13 @@ template <int L, int C> __host__ __device__ constexpr int pass_threads() { retur
 
 // One work-group of a pass launch: the tile this work-group owns, with the
 // work-group's own LDS.
-template <int L, int C, int V, class IO>
+template <int L, int C, int V, class IO, int MODES = 0>
 __device__ __forceinline__ void fft_pass_body(const rk_pass_args& a, const IO& io) {
     constexpr int LP = tile_pitch<L, C>();
     __shared__ float2 lds[tile_split<L, C>() ? (C * LP + 1) / 2 : C * LP];
     const rk_u64 tile = xcd_tile(a) + a.tile_base;
     if (tile >= a.n_tiles) return;
-    fft_pass_tile<L, C, V, IO>(a, io, lds, tile, (int)threadIdx.x);
+    fft_pass_tile<L, C, V, IO, MODES>(a, io, lds, tile, (int)threadIdx.x);
 }
 
 }   // namespace rkfft

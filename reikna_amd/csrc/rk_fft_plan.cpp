@@ -558,8 +558,33 @@ static bool stream_pass(const PassPlan& pp) {
     return bytes > stream_min_bytes();
 }
 
+// Kernels with the pass's load/store modes compiled in (RK_FIXED_MODES=0
+// turns them off).
+static bool use_fixed_modes() {
+    static const bool v = [] {
+        const char* e = getenv("RK_FIXED_MODES");
+        return !(e && *e && atoi(e) == 0);
+    }();
+    return v;
+}
+
 static int find_kernel(PassPlan& pp, uint32_t flags) {
+    pp.fixed_modes = false;
     if (pp.stream) {
+        // a twin with this pass's load/store modes compiled in, when there is one
+        const rk_u32 lm = pp.args.load_mode, sm = pp.args.store_mode;
+        const int mbit = !use_fixed_modes() || lm != RK_MODE_DIRECT ? 0
+                         : sm == RK_MODE_DIRECT ? VAR_MODE_DD
+                         : sm == RK_MODE_FLAT ? VAR_MODE_DF : 0;
+        if (mbit)
+            for (const KernelEntry& e : catalog())
+                if (e.L == pp.L && e.C == pp.C && e.variant == (pp.variant | VAR_STREAM | mbit)) {
+                    pp.aot = e.func;
+                    pp.threads = e.threads;
+                    pp.lds_bytes = e.lds_bytes;
+                    pp.fixed_modes = true;
+                    return RK_OK;
+                }
         for (const KernelEntry& e : catalog())
             if (e.L == pp.L && e.C == pp.C && e.variant == (pp.variant | VAR_STREAM)) {
                 pp.aot = e.func;
@@ -1002,6 +1027,7 @@ extern "C" int rk_fft_plan_pass_info(const rk_fft_plan* plan, int pass, rk_fft_p
     info->kind = pp.kind;
     info->load_mode = (int)pp.args.load_mode;
     info->store_mode = (int)pp.args.store_mode;
+    info->fixed_modes = pp.fixed_modes ? 1 : 0;
     info->src = pp.src;
     info->dst = pp.dst;
     info->axis = pp.axis;
@@ -1027,6 +1053,7 @@ extern "C" int rk_fft_plan_set_pass_kernel(rk_fft_plan* plan, int pass, void* fu
     plan->passes[pass].jit = (hipFunction_t)function;
     plan->passes[pass].aot = nullptr;
     plan->passes[pass].user_kernel = true;
+    plan->passes[pass].fixed_modes = false;
     return RK_OK;
 }
 

@@ -48,6 +48,8 @@ enum {
     VAR_BLUE = 2 | (2 << 2) | (1 << 4),
     VAR_R2C = 1 << 6,   // batch kernel only: the real-to-complex combine pass
     VAR_STREAM = 1 << 7,   // catalogue entries: the kernel uses StreamIO (RK_VAR_STREAM)
+    VAR_MODE_DD = 1 << 8,  // catalogue entries: DIRECT/DIRECT load/store modes compiled in (RK_VAR_MODE_DD)
+    VAR_MODE_DF = 1 << 9,  // catalogue entries: DIRECT/FLAT load/store modes compiled in (RK_VAR_MODE_DF)
 };
 
 // All ahead-of-time kernels compiled into this library.
@@ -116,6 +118,7 @@ struct PassPlan {
     bool user_kernel = false;       // replaced with rk_fft_plan_set_pass_kernel
     bool is_first = false, is_last = false;
     bool stream = false;            // nontemporal loads/stores (StreamIO kernel)
+    bool fixed_modes = false;       // the kernel has this pass's load/store modes compiled in
     uint64_t grid = 0;
     int axis = 0;
     int kind = 0;

@@ -38,6 +38,7 @@ class PassInfo:
     jit: bool
     variant: int
     stream: bool = False   # nontemporal loads/stores (transform larger than the Infinity Cache)
+    fixed_modes: bool = False   # the kernel has the pass's load/store modes compiled in
 
 
 def _element_layout(meta: ArrayMetadata) -> tuple[list[int], int]:
@@ -108,6 +109,7 @@ class NativeFftPlan:
                     post_ops=info.post_ops, n_inst=info.n_inst, grid=info.grid, axis_len=info.axis_len,
                     Ns=info.Ns, lds_bytes=info.lds_bytes, jit=bool(info.jit), variant=info.variant,
                     stream=bool(info.stream),
+                    fixed_modes=bool(info.fixed_modes),
                 )
             )
         return out

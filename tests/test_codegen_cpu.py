@@ -148,6 +148,22 @@ def test_plan_streaming_accesses(vdev):
     assert not any(p.stream for p in passes((4096, 65521), (1,), dev=vdev) if p.variant & 2)
 
 
+def test_plan_fixed_mode_kernels(vdev):
+    # the column passes of large multi-pass transforms whose lengths hold a
+    # factor 3 or 5 run kernels with their load/store modes compiled in;
+    # power-of-two passes and passes with other modes keep the generic kernel
+    for shape in ((4096, 12288), (512, 3 * 2**16), (256, 5 * 2**16)):
+        ps = passes(shape, (1,), dev=vdev)
+        assert len(ps) >= 2 and all(p.stream for p in ps)
+        assert [p.fixed_modes for p in ps] == [p.L & (p.L - 1) != 0 for p in ps]
+        assert any(p.fixed_modes for p in ps)
+        assert (ps[0].load_mode, ps[0].store_mode) == ("direct", "flat")
+        assert all((p.load_mode, p.store_mode) == ("direct", "direct") for p in ps[1:])
+    assert not any(p.fixed_modes for p in passes((64, 2**20), (1,), dev=vdev))
+    assert not any(p.fixed_modes for p in passes((1024, 1024), (1,), dev=vdev))
+    assert not any(p.fixed_modes for p in passes((256, 256, 256), (0, 1, 2), dev=vdev))
+
+
 def test_plan_xcd_groups_for_narrow_column_tiles(vdev):
     # a 4096-point strided axis: column tiles of 4 instances (the pass is planned;
     # its XCD grouping is a launch detail checked on the GPU)

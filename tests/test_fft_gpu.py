@@ -232,6 +232,23 @@ def test_strided_input_and_output(queue):
     assert oracle.rel_l2(out.get(queue), oracle.fftn(data, (1,))) <= oracle.tolerance((64, 1024), (1,))
 
 
+@pytest.mark.parametrize(
+    "shape,axes",
+    [((512, 3 * 2**16), (1,)), ((256, 5 * 2**16), (1,)), ((2**16, 512), (0,)), ((2**14, 4096), (0, 1))],
+    ids=lambda v: str(v),
+)
+def test_large_streaming_passes(queue, shape, axes):
+    """Transforms larger than the Infinity Cache: streaming multi-pass chains,
+    whose column passes of lengths with a factor 3 or 5 run kernels with the
+    load/store modes compiled in."""
+    data = oracle.test_array(shape, seed=sum(shape))
+    fwd, fft = run_fft(queue, data, axes, inplace=False)
+    assert any(p.stream for p in fft.native_plan.passes)
+    assert oracle.rel_l2(fwd, oracle.fftn(data, axes)) <= oracle.tolerance(shape, axes)
+    inv, _ = run_fft(queue, data, axes, inverse=True, inplace=True)
+    assert oracle.rel_l2(inv, oracle.ifftn(data, axes)) <= oracle.tolerance(shape, axes)
+
+
 @pytest.mark.parametrize("shape", [(4096, 40), (2048, 100), (16384, 5), (3 * 2**12, 7), (1024, 3, 5)])
 def test_long_strided_axis(queue, shape):
     """Axis 0 long and strided: column tiles narrower than 16 instances, sent to
