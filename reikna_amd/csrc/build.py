@@ -168,6 +168,12 @@ def catalog_entries() -> list[tuple[int, int, int]]:
                 ents.add((length, c, v | VAR_MODE_DD))
             elif v == VAR_TWIDDLE | VAR_STREAM:
                 ents.add((length, c, v | VAR_MODE_DD))
+    # single-pass row tiles of such lengths (one thread per R0 elements of an
+    # instance, at least 32 of them: shorter rows are streamed flat): the
+    # 1000-point tile holds 80 VGPRs with every mode and 62 with DIRECT/DIRECT
+    for length in catalog_lengths():
+        if length & (length - 1) and length <= STREAM_MAX and threads_per_instance(length) >= 32:
+            ents.add((length, tile_c_row(length), VAR_PLAIN | VAR_MODE_DD))
     return sorted(ents)
 
 

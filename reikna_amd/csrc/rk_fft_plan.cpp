@@ -594,6 +594,15 @@ static int find_kernel(PassPlan& pp, uint32_t flags) {
             }
         pp.stream = false;   // no streaming twin: the plain kernel
     }
+    if (use_fixed_modes() && pp.args.load_mode == RK_MODE_DIRECT && pp.args.store_mode == RK_MODE_DIRECT)
+        for (const KernelEntry& e : catalog())
+            if (e.L == pp.L && e.C == pp.C && e.variant == (pp.variant | VAR_MODE_DD)) {
+                pp.aot = e.func;
+                pp.threads = e.threads;
+                pp.lds_bytes = e.lds_bytes;
+                pp.fixed_modes = true;
+                return RK_OK;
+            }
     for (const KernelEntry& e : catalog())
         if (e.L == pp.L && e.C == pp.C && e.variant == pp.variant) {
             pp.aot = e.func;

@@ -161,6 +161,11 @@ def test_plan_fixed_mode_kernels(vdev):
         assert all((p.load_mode, p.store_mode) == ("direct", "direct") for p in ps[1:])
     assert not any(p.fixed_modes for p in passes((64, 2**20), (1,), dev=vdev))
     assert not any(p.fixed_modes for p in passes((1024, 1024), (1,), dev=vdev))
+    # single-pass rows of 3/5-smooth lengths with >= 32 threads per instance
+    for n in (1000, 2000, 1536, 4000):
+        ps = passes((64, n), (1,), dev=vdev)
+        assert [(p.load_mode, p.store_mode, p.fixed_modes) for p in ps] == [("direct", "direct", True)]
+    assert not passes((64, 96), (1,), dev=vdev)[0].fixed_modes   # short rows stream flat
     assert not any(p.fixed_modes for p in passes((256, 256, 256), (0, 1, 2), dev=vdev))
 
 
