@@ -174,6 +174,15 @@ def catalog_entries() -> list[tuple[int, int, int]]:
     for length in catalog_lengths():
         if length & (length - 1) and length <= STREAM_MAX and threads_per_instance(length) >= 32:
             ents.add((length, tile_c_row(length), VAR_PLAIN | VAR_MODE_DD))
+    # single-kernel Bluestein tiles up to M = 1024 with DIRECT/DIRECT compiled
+    # in and bounded to three waves per SIMD (at most 168 VGPRs, a few
+    # spilled): 8192 x 251 (M = 512) 24.1 against 28.5 us, 4096 x 509 and
+    # 16384 x 61 4 % faster; from M = 2048 the bound costs more than the
+    # occupancy gives (4096 x 997 69 against 58 us), so those keep the generic
+    # kernel
+    for length, c, v in list(ents):
+        if v == VAR_BLUE and length <= 1024 and c * threads_per_instance(length) <= 256:
+            ents.add((length, c, v | VAR_MODE_DD))
     return sorted(ents)
 
 
@@ -264,7 +273,8 @@ def generate_catalog(out_dir: Path) -> list[Path]:
             '#include "rk_internal.hpp"',
             f"namespace rk_catalog_{k} {{",
             "template <int L, int C, int V>",
-            "__global__ void __launch_bounds__((rkfft::pass_threads<L, C>()))",
+            "__global__ void __launch_bounds__((rkfft::pass_threads<L, C>()),",
+            "    ((V & RK_VAR_MODE_DD) && (V & 16) && rkfft::pass_threads<L, C>() <= 256) ? 3 : 1)",
             "rk_fft_pass(rk_pass_args a) {",
             "    constexpr int VB = V & (RK_VAR_STREAM - 1);",
             "    constexpr int M = (V & RK_VAR_MODE_DD) ? RK_MODES(RK_MODE_DIRECT, RK_MODE_DIRECT)",
