@@ -160,13 +160,15 @@ def catalog_entries() -> list[tuple[int, int, int]]:
     # every mode, 70-83 with two) and run 2-5 % faster with the modes fixed;
     # power-of-two tiles are LDS-bound at four waves per SIMD either way and
     # measured 1-2 % slower with them.
+    # The same for the plain (non-streaming) column passes of such lengths:
+    # strided axes of multi-axis transforms and chains below the cache size.
     for length, c, v in list(ents):
         pow2 = length & (length - 1) == 0
-        if not pow2 and length <= FACTOR_MAX and c == tile_c_col(length) and v & VAR_STREAM:
-            if v == VAR_PLAIN | VAR_STREAM:
+        if not pow2 and length <= FACTOR_MAX and c == tile_c_col(length) and v & ~VAR_STREAM in (VAR_PLAIN, VAR_TWIDDLE):
+            if v & ~VAR_STREAM == VAR_PLAIN:
                 ents.add((length, c, v | VAR_MODE_DF))
                 ents.add((length, c, v | VAR_MODE_DD))
-            elif v == VAR_TWIDDLE | VAR_STREAM:
+            else:
                 ents.add((length, c, v | VAR_MODE_DD))
     # single-pass row tiles of such lengths (one thread per R0 elements of an
     # instance, at least 32 of them: shorter rows are streamed flat): the

@@ -568,14 +568,20 @@ static bool use_fixed_modes() {
     return v;
 }
 
+// Catalogue bit of the twin with this pass's load/store modes compiled in
+// (0: the pass's modes have no such twins).
+static int fixed_mode_bit(const PassPlan& pp) {
+    if (!use_fixed_modes() || pp.args.load_mode != RK_MODE_DIRECT) return 0;
+    if (pp.args.store_mode == RK_MODE_DIRECT) return VAR_MODE_DD;
+    if (pp.args.store_mode == RK_MODE_FLAT) return VAR_MODE_DF;
+    return 0;
+}
+
 static int find_kernel(PassPlan& pp, uint32_t flags) {
     pp.fixed_modes = false;
+    const int mbit = fixed_mode_bit(pp);
     if (pp.stream) {
         // a twin with this pass's load/store modes compiled in, when there is one
-        const rk_u32 lm = pp.args.load_mode, sm = pp.args.store_mode;
-        const int mbit = !use_fixed_modes() || lm != RK_MODE_DIRECT ? 0
-                         : sm == RK_MODE_DIRECT ? VAR_MODE_DD
-                         : sm == RK_MODE_FLAT ? VAR_MODE_DF : 0;
         if (mbit)
             for (const KernelEntry& e : catalog())
                 if (e.L == pp.L && e.C == pp.C && e.variant == (pp.variant | VAR_STREAM | mbit)) {
@@ -594,9 +600,9 @@ static int find_kernel(PassPlan& pp, uint32_t flags) {
             }
         pp.stream = false;   // no streaming twin: the plain kernel
     }
-    if (use_fixed_modes() && pp.args.load_mode == RK_MODE_DIRECT && pp.args.store_mode == RK_MODE_DIRECT)
+    if (mbit)
         for (const KernelEntry& e : catalog())
-            if (e.L == pp.L && e.C == pp.C && e.variant == (pp.variant | VAR_MODE_DD)) {
+            if (e.L == pp.L && e.C == pp.C && e.variant == (pp.variant | mbit)) {
                 pp.aot = e.func;
                 pp.threads = e.threads;
                 pp.lds_bytes = e.lds_bytes;

@@ -249,6 +249,23 @@ def test_large_streaming_passes(queue, shape, axes):
     assert oracle.rel_l2(inv, oracle.ifftn(data, axes)) <= oracle.tolerance(shape, axes)
 
 
+@pytest.mark.parametrize(
+    "shape,axes",
+    [((192, 192, 192), (0, 1, 2)), ((96, 96, 96), (0, 1, 2)), ((64, 3 * 2**13), (1,)), ((64, 4000), (1,)),
+     ((4096, 1000), (1,))],
+    ids=lambda v: str(v),
+)
+def test_fixed_mode_passes(queue, shape, axes):
+    """Column passes and single-pass rows of lengths with a factor 3 or 5 run
+    kernels with their load/store modes compiled in."""
+    data = oracle.test_array(shape, seed=sum(shape) + 1)
+    fwd, fft = run_fft(queue, data, axes, inplace=False)
+    assert any(p.fixed_modes for p in fft.native_plan.passes)
+    assert oracle.rel_l2(fwd, oracle.fftn(data, axes)) <= oracle.tolerance(shape, axes)
+    inv, _ = run_fft(queue, data, axes, inverse=True, inplace=True)
+    assert oracle.rel_l2(inv, oracle.ifftn(data, axes)) <= oracle.tolerance(shape, axes)
+
+
 @pytest.mark.parametrize("shape", [(4096, 40), (2048, 100), (16384, 5), (3 * 2**12, 7), (1024, 3, 5)])
 def test_long_strided_axis(queue, shape):
     """Axis 0 long and strided: column tiles narrower than 16 instances, sent to
