@@ -136,6 +136,7 @@ STREAM_MAX = 4096                    # streaming kernels for plain/twiddle passe
 BLUE_SINGLE_MAX = 8192               # longest single-kernel Bluestein convolution (rk_fft_plan.cpp)
 VAR_MODE_DD = 1 << 8                 # load/store modes DIRECT/DIRECT compiled in (RK_VAR_MODE_DD)
 VAR_MODE_DF = 1 << 9                 # load/store modes DIRECT/FLAT compiled in (RK_VAR_MODE_DF)
+VAR_MODE_FF = 1 << 10                # load/store modes FLAT/FLAT compiled in (RK_VAR_MODE_FF)
 FACTOR_MAX = 256                     # longest pass of a multi-pass chain (rk_fft_plan.cpp factor_max)
 
 
@@ -176,6 +177,12 @@ def catalog_entries() -> list[tuple[int, int, int]]:
     for length in catalog_lengths():
         if length & (length - 1) and length <= STREAM_MAX and threads_per_instance(length) >= 32:
             ents.add((length, tile_c_row(length), VAR_PLAIN | VAR_MODE_DD))
+    # short rows of such lengths (under 32 threads per instance) stream their
+    # tiles flat on both sides: the 96-, 160- and 192-point row tiles hold
+    # 140-160 VGPRs with every mode and 120-124 with FLAT/FLAT
+    for length in catalog_lengths():
+        if length & (length - 1) and length >= 8 and threads_per_instance(length) < 32:
+            ents.add((length, tile_c_row(length), VAR_PLAIN | VAR_MODE_FF))
     # single-kernel Bluestein tiles up to M = 1024 with DIRECT/DIRECT compiled
     # in and bounded to three waves per SIMD (at most 168 VGPRs, a few
     # spilled): 8192 x 251 (M = 512) 24.1 against 28.5 us, 4096 x 509 and
@@ -254,7 +261,7 @@ def generate_batch_cases(out_dir: Path) -> list[Path]:
 
 
 def variant_cost(v: int) -> float:
-    v &= ~(VAR_STREAM | VAR_MODE_DD | VAR_MODE_DF)
+    v &= ~(VAR_STREAM | VAR_MODE_DD | VAR_MODE_DF | VAR_MODE_FF)
     return 2.0 if v & (1 << 4) else 1.3 if v else 1.0
 
 
@@ -280,7 +287,8 @@ def generate_catalog(out_dir: Path) -> list[Path]:
             "rk_fft_pass(rk_pass_args a) {",
             "    constexpr int VB = V & (RK_VAR_STREAM - 1);",
             "    constexpr int M = (V & RK_VAR_MODE_DD) ? RK_MODES(RK_MODE_DIRECT, RK_MODE_DIRECT)",
-            "                    : (V & RK_VAR_MODE_DF) ? RK_MODES(RK_MODE_DIRECT, RK_MODE_FLAT) : 0;",
+            "                    : (V & RK_VAR_MODE_DF) ? RK_MODES(RK_MODE_DIRECT, RK_MODE_FLAT)",
+            "                    : (V & RK_VAR_MODE_FF) ? RK_MODES(RK_MODE_FLAT, RK_MODE_FLAT) : 0;",
             "    if constexpr ((V & RK_VAR_STREAM) != 0)",
             "        rkfft::fft_pass_body<L, C, VB, rkfft::StreamIO, M>(a, rkfft::StreamIO{});",
             "    else",

@@ -435,12 +435,13 @@ struct CoherentIO {
 // uses StreamIO instead of PlainIO.
 #define RK_VAR_STREAM 128
 // Bits of a catalogue kernel's variant number: the kernel has the pass's load
-// and store modes compiled in (DIRECT/DIRECT, DIRECT/FLAT) instead of reading
+// and store modes compiled in (DIRECT/DIRECT, DIRECT/FLAT, FLAT/FLAT) instead of reading
 // them from the pass arguments.  The run-time switch keeps the code of every
 // mode alive in one kernel: a 96-point tile holds 146 VGPRs with it and 75-82
 // with its modes fixed.
 #define RK_VAR_MODE_DD 256
 #define RK_VAR_MODE_DF 512
+#define RK_VAR_MODE_FF 1024
 // MODES template argument of fft_pass_tile: 0 reads the modes from the pass
 // arguments, RK_MODES(load, store) compiles them in.
 #define RK_MODES(load, store) (16 | (int)(load) | ((int)(store) << 2))

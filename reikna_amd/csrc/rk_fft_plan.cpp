@@ -571,9 +571,11 @@ static bool use_fixed_modes() {
 // Catalogue bit of the twin with this pass's load/store modes compiled in
 // (0: the pass's modes have no such twins).
 static int fixed_mode_bit(const PassPlan& pp) {
-    if (!use_fixed_modes() || pp.args.load_mode != RK_MODE_DIRECT) return 0;
-    if (pp.args.store_mode == RK_MODE_DIRECT) return VAR_MODE_DD;
-    if (pp.args.store_mode == RK_MODE_FLAT) return VAR_MODE_DF;
+    if (!use_fixed_modes()) return 0;
+    const rk_u32 lm = pp.args.load_mode, sm = pp.args.store_mode;
+    if (lm == RK_MODE_DIRECT && sm == RK_MODE_DIRECT) return VAR_MODE_DD;
+    if (lm == RK_MODE_DIRECT && sm == RK_MODE_FLAT) return VAR_MODE_DF;
+    if (lm == RK_MODE_FLAT && sm == RK_MODE_FLAT) return VAR_MODE_FF;
     return 0;
 }
 

@@ -50,6 +50,7 @@ enum {
     VAR_STREAM = 1 << 7,   // catalogue entries: the kernel uses StreamIO (RK_VAR_STREAM)
     VAR_MODE_DD = 1 << 8,  // catalogue entries: DIRECT/DIRECT load/store modes compiled in (RK_VAR_MODE_DD)
     VAR_MODE_DF = 1 << 9,  // catalogue entries: DIRECT/FLAT load/store modes compiled in (RK_VAR_MODE_DF)
+    VAR_MODE_FF = 1 << 10, // catalogue entries: FLAT/FLAT load/store modes compiled in (RK_VAR_MODE_FF)
 };
 
 // All ahead-of-time kernels compiled into this library.

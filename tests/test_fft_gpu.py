@@ -252,7 +252,7 @@ def test_large_streaming_passes(queue, shape, axes):
 @pytest.mark.parametrize(
     "shape,axes",
     [((192, 192, 192), (0, 1, 2)), ((96, 96, 96), (0, 1, 2)), ((64, 3 * 2**13), (1,)), ((64, 4000), (1,)),
-     ((4096, 1000), (1,))],
+     ((4096, 1000), (1,)), ((8192, 96), (1,)), ((4096, 160), (1,))],
     ids=lambda v: str(v),
 )
 def test_fixed_mode_passes(queue, shape, axes):
