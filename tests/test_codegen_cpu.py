@@ -165,7 +165,10 @@ def test_plan_fixed_mode_kernels(vdev):
     for n in (1000, 2000, 1536, 4000):
         ps = passes((64, n), (1,), dev=vdev)
         assert [(p.load_mode, p.store_mode, p.fixed_modes) for p in ps] == [("direct", "direct", True)]
-    assert not passes((64, 96), (1,), dev=vdev)[0].fixed_modes   # short rows stream flat
+    # short rows stream their tiles flat on both sides, with FLAT/FLAT compiled in
+    ps = passes((64, 96), (1,), dev=vdev)
+    assert [(p.load_mode, p.store_mode, p.fixed_modes) for p in ps] == [("flat", "flat", True)]
+    assert not passes((64, 256), (1,), dev=vdev)[0].fixed_modes   # power of two: generic kernel
     assert not any(p.fixed_modes for p in passes((256, 256, 256), (0, 1, 2), dev=vdev))
 
 
