@@ -22,6 +22,7 @@ PureParallel so that transformations still run (:647-662).
 from __future__ import annotations
 
 import ctypes
+import os
 from collections.abc import Callable, Iterable
 from typing import Any
 
@@ -42,6 +43,17 @@ def _meta(arr_t: Any) -> ArrayMetadata:
     if hasattr(arr_t, "as_array_metadata"):
         return arr_t.as_array_metadata()
     return ArrayMetadata.from_arraylike(arr_t)
+
+
+# RK_MODE_* codes of fft_args.h by the pass-info names of native.LOAD_MODES
+_MODE_CODES = {"direct": 0, "column": 1, "flat": 2, "row": 3}
+
+
+def _fixed_modes() -> bool:
+    """Whether kernels get their pass's load/store modes compiled in (the
+    library's RK_FIXED_MODES switch: 0 turns it off)."""
+    v = os.environ.get("RK_FIXED_MODES", "")
+    return not (v.strip().lstrip("-").isdigit() and int(v) == 0)
 
 
 def _ptr(val: Any) -> int:
@@ -204,12 +216,15 @@ class _FusedFft:
                 " rk_i64 addr, float2 v) const {\n" + store_body + "    }\n"
                 "};\n"
             )
+            # the pass's load/store modes compiled in (fft_device.hpp RK_MODES)
+            modes = (f"RK_MODES({_MODE_CODES[info.load_mode]}, {_MODE_CODES[info.store_mode]})"
+                     if _fixed_modes() else "0")
             parts.append(
                 f'extern "C" __global__ void __launch_bounds__({info.threads}) '
                 "rk_fused_pass(rk_pass_args a, rk_user_blob lb, rk_user_blob sb) {\n"
                 "    const RkLoadArgs& la = *reinterpret_cast<const RkLoadArgs*>(lb.b);\n"
                 "    const RkStoreArgs& sa = *reinterpret_cast<const RkStoreArgs*>(sb.b);\n"
-                f"    rkfft::fft_pass_body<{info.L}, {info.C}, {info.variant}>(a, RkFusedIO{{la, sa}});\n"
+                f"    rkfft::fft_pass_body<{info.L}, {info.C}, {info.variant}, RkFusedIO, {modes}>(a, RkFusedIO{{la, sa}});\n"
                 "}\n"
             )
             return "".join(parts)

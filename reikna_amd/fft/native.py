@@ -13,7 +13,7 @@ import numpy
 from .. import _lib
 from ..runtime import ArrayMetadata
 
-LOAD_MODES = {0: "direct", 1: "column", 2: "flat"}
+LOAD_MODES = {0: "direct", 1: "column", 2: "flat", 3: "row"}
 BUFFERS = {0: "in", 1: "out", 2: "ws0", 3: "ws1"}
 
 
