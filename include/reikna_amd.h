@@ -322,8 +322,11 @@ typedef struct {
     uint32_t spin_limit;   /* polls before a dependency wait of the batch kernel gives
                               up and flags its launch (0: 2^24; small values are for
                               tests of the failure path) */
-    uint32_t _reserved;
+    uint32_t fault_inject; /* 0; RK_SERVER_FAULT_* bits make the next such step fail
+                              (tests of the failure path) */
 } rk_server_config;
+
+#define RK_SERVER_FAULT_SLOT_ALLOC 1u   /* the next growth of a slot's buffer fails */
 
 typedef struct {
     uint64_t id;           /* caller's tag, unique among unreleased requests */
@@ -340,21 +343,23 @@ typedef struct {
     void* out;             /* device output, valid until rk_server_release(id) */
     uint64_t t_arrival_ns, t_admitted_ns, t_done_ns;
     int32_t status;        /* 0; 1 if a dependency wait of its launch gave up (the
-                              output is not valid; the server has reset its state) */
+                              output is not valid; the server has reset its state);
+                              2 if its launch could not be enqueued (not run; the
+                              rk_server_progress call that tried returned the error) */
     int32_t _reserved;
 } rk_completion;
 
 typedef struct {
     int64_t pending, in_flight_requests, in_flight_launches, completed, launches, templates;
     size_t budget_used, budget_bytes;
-    int64_t failed_launches;   /* launches whose completions carry status 1 */
+    int64_t failed_launches;   /* launches whose completions carry status 1 or 2 */
 } rk_server_stats_t;
 
 int rk_server_create(const rk_server_config* config, rk_server** server);
 /* Queue requests (they wait for memory and a slot); RK_ERR_BUDGET for a
    request that can never fit in the budget, RK_ERR_INVALID for an id that is
-   still in use (submitted and not released).  Requests before the failing
-   one stay queued. */
+   still in use (submitted and not released) or given twice.  All requests are
+   checked first: on an error none of them is queued. */
 int rk_server_submit(rk_server* server, const rk_request* requests, int n);
 /* Reap finished launches, admit and launch queued requests, and return up to
    max_done completions (never blocks). */

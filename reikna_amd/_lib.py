@@ -41,7 +41,7 @@ class ServerConfig(ctypes.Structure):
         ("mode", ctypes.c_uint32),
         ("plain_min_elements", ctypes.c_int64),
         ("spin_limit", ctypes.c_uint32),
-        ("_reserved", ctypes.c_uint32),
+        ("fault_inject", ctypes.c_uint32),
     ]
 
 
@@ -85,6 +85,7 @@ class ServerStats(ctypes.Structure):
 
 RK_SERVER_BATCHED = 0
 RK_SERVER_PLAIN = 1
+RK_SERVER_FAULT_SLOT_ALLOC = 1
 
 
 class RkError(RuntimeError):

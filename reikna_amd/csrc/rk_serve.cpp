@@ -260,10 +260,21 @@ static int get_template(rk_server* s, const rk_request& r, const Template** out)
 
 extern "C" int rk_server_submit(rk_server* s, const rk_request* reqs, int n) {
     if (!s || (n > 0 && !reqs) || n < 0) return fail(RK_ERR_INVALID, "rk_server_submit: bad arguments");
+    // every request is checked (and its template built) before any is
+    // queued: a batch is accepted whole or not at all
+    std::vector<Request> batch;
+    batch.reserve((size_t)n);
+    std::unordered_set<uint64_t> ids;
     for (int i = 0; i < n; ++i) {
         const rk_request& r = reqs[i];
         if ((r.ndim != 1 && r.ndim != 2) || r.count < 1 || r.shape[0] < 2 || (r.ndim == 2 && r.shape[1] < 2) || !r.in)
             return fail(RK_ERR_INVALID, "rk_server_submit: request %d: bad shape/count/input", i);
+        if (s->live.count(r.id))
+            return fail(RK_ERR_INVALID, "rk_server_submit: request %d: id %llu is still in use", i,
+                        (unsigned long long)r.id);
+        if (!ids.insert(r.id).second)
+            return fail(RK_ERR_INVALID, "rk_server_submit: request %d: id %llu appears twice in the batch", i,
+                        (unsigned long long)r.id);
         Request q;
         q.r = r;
         int rc = get_template(s, r, &q.t);
@@ -275,11 +286,12 @@ extern "C" int rk_server_submit(rk_server* s, const rk_request* reqs, int n) {
         if (need > s->arena.size())
             return fail(RK_ERR_BUDGET, "rk_server_submit: request %d needs %zu bytes, the budget is %zu", i, need,
                         s->arena.size());
-        if (s->live.count(r.id))
-            return fail(RK_ERR_INVALID, "rk_server_submit: request %d: id %llu is still in use", i,
-                        (unsigned long long)r.id);
-        if (q.r.t_arrival_ns == 0) q.r.t_arrival_ns = now_ns();
-        s->live.insert(r.id);
+        batch.push_back(q);
+    }
+    const uint64_t t = now_ns();
+    for (Request& q : batch) {
+        if (q.r.t_arrival_ns == 0) q.r.t_arrival_ns = t;
+        s->live.insert(q.r.id);
         s->pending.push_back(q);
     }
     return RK_OK;
@@ -343,16 +355,32 @@ static int launch_batched(rk_server* s, Slot& sl) {
     const size_t line_bytes = RK_BATCH_LINE * sizeof(rk_u32);
     if (sl.passes.size() > sl.cap_passes || total > sl.cap_tiles || lines > sl.cap_lines) {
         RK_HIP(hipStreamSynchronize(sl.stream));
-        if (sl.dev) RK_HIP(hipFree(sl.dev));
-        sl.cap_passes = std::max<size_t>(sl.passes.size() * 2, 1024);
-        sl.cap_tiles = std::max<size_t>(total * 2, 16384);
-        sl.cap_lines = std::max<size_t>(lines * 2, 1024);
-        const size_t bytes = sl.cap_passes * sizeof(rk_batch_pass) + sl.cap_tiles * 4 + sl.cap_lines * line_bytes + 256;
-        RK_HIP(hipMalloc(&sl.dev, bytes));
-        sl.bytes = bytes;
+        const size_t cap_passes = std::max<size_t>(sl.passes.size() * 2, 1024);
+        const size_t cap_tiles = std::max<size_t>(total * 2, 16384);
+        const size_t cap_lines = std::max<size_t>(lines * 2, 1024);
+        const size_t bytes = cap_passes * sizeof(rk_batch_pass) + cap_tiles * 4 + cap_lines * line_bytes + 256;
+        // the new block is allocated and cleared before the old one goes, and
+        // the capacities change only with it: a failure leaves the slot as it
+        // was (its old block, if any, still valid for its old capacities)
+        void* fresh = nullptr;
+        if (s->cfg.fault_inject & RK_SERVER_FAULT_SLOT_ALLOC) {
+            s->cfg.fault_inject &= ~RK_SERVER_FAULT_SLOT_ALLOC;
+            return fail(RK_ERR_HIP, "rk_server: slot buffer of %zu bytes: allocation failed (injected)", bytes);
+        }
+        RK_HIP(hipMalloc(&fresh, bytes));
         // on the slot's stream: ordered before the descriptor uploads below
         // (the slot's stream does not wait for the null stream)
-        RK_HIP(hipMemsetAsync(sl.dev, 0, bytes, sl.stream));
+        const hipError_t me = hipMemsetAsync(fresh, 0, bytes, sl.stream);
+        if (me != hipSuccess) {
+            hipFree(fresh);
+            return fail(RK_ERR_HIP, "rk_server: clearing a slot buffer: %s", hipGetErrorString(me));
+        }
+        if (sl.dev) hipFree(sl.dev);   // idle: the stream was synchronised above
+        sl.dev = fresh;
+        sl.bytes = bytes;
+        sl.cap_passes = cap_passes;
+        sl.cap_tiles = cap_tiles;
+        sl.cap_lines = cap_lines;
     }
     char* base = (char*)sl.dev;
     rk_batch_work w;
@@ -440,7 +468,15 @@ extern "C" int rk_server_progress(rk_server* s, rk_completion* done, int max_don
         if (sl.reqs.empty()) break;   // memory is full: wait for a launch to finish
         sl.busy = true;
         int rc = sl.plain ? launch_plain(s, sl) : launch_batched(s, sl);
-        if (rc != RK_OK) return rc;
+        if (rc != RK_OK) {
+            // whatever part of the launch was enqueued ends before the
+            // requests' memory is reused; the requests complete with status
+            // 2 (not run), their workspaces are released and the slot is free
+            hipStreamSynchronize(sl.stream);
+            s->failed++;
+            finish_slot(s, sl, 2);
+            return rc;
+        }
     }
     const int n = std::min<int>(max_done, (int)s->finished.size());
     for (int i = 0; i < n; ++i) done[i] = s->finished[i];

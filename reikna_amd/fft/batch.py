@@ -20,7 +20,7 @@ from collections.abc import Sequence
 from dataclasses import dataclass
 
 from .. import _lib
-from ..runtime import Array, Buffer, Device, Queue, VirtualDevice
+from ..runtime import Array, Buffer, Device, Queue, VirtualDevice, require_dense
 
 __all__ = ["BatchFFT", "BatchItem", "BatchOptions"]
 
@@ -191,6 +191,9 @@ class BatchFFT:
         have = input.size // 2 if real_in else input.size
         if have < self.in_elements:
             raise ValueError(f"input holds {have} complex64 elements' worth, the batch needs {self.in_elements}")
+        # the kernel walks both buffers as dense runs from their data pointers
+        require_dense("output", output, 8 * self.out_elements)
+        require_dense("input", input, 8 * self.in_elements)
         _lib.call(
             "rk_batch_execute", self._h, ctypes.c_void_p(output.data_ptr), ctypes.c_void_p(input.data_ptr),
             int(bool(inverse)), ctypes.c_void_p(self._workspace() or None), queue.stream,

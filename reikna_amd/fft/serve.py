@@ -23,7 +23,7 @@ from dataclasses import dataclass, field
 import numpy
 
 from .. import _lib
-from ..runtime import Array, Device, _ForeignBuffer
+from ..runtime import Array, Device, _ForeignBuffer, require_dense
 from ..runtime import ArrayMetadata
 
 __all__ = ["FFTServer", "RequestSpec", "make_specs", "run_stream", "sample_shapes"]
@@ -54,7 +54,10 @@ class Completion:
     t_arrival_ns: int
     t_admitted_ns: int
     t_done_ns: int
-    status: int = 0   # 1: a dependency wait of its launch gave up, the output is not valid
+    # 1: a dependency wait of its launch gave up; 2: its launch could not be
+    # enqueued (the progress() call that tried raised); the output is not
+    # valid in either case
+    status: int = 0
 
     @property
     def latency_ms(self) -> float:
@@ -67,6 +70,7 @@ def _check_spec(rid: int, spec: RequestSpec) -> None:
     if spec.input.dtype != numpy.complex64 or spec.input.size < need:
         raise ValueError(f"request {rid}: input must be complex64 with at least {need} elements "
                          f"(got {spec.input.dtype} x {spec.input.size})")
+    require_dense(f"request {rid} input", spec.input, 8 * need)
 
 
 class FFTServer:
@@ -79,6 +83,7 @@ class FFTServer:
         max_requests: int = 1024,
         plain_min_elements: int = 0,
         spin_limit: int = 0,
+        fault_inject: int = 0,
     ):
         if mode not in ("batched", "plain"):
             raise ValueError(f"unknown server mode {mode!r}")
@@ -92,6 +97,7 @@ class FFTServer:
         cfg.mode = _lib.RK_SERVER_BATCHED if mode == "batched" else _lib.RK_SERVER_PLAIN
         cfg.plain_min_elements = int(plain_min_elements)
         cfg.spin_limit = int(spin_limit)
+        cfg.fault_inject = int(fault_inject)
         h = ctypes.c_void_p()
         _lib.call("rk_server_create", ctypes.byref(cfg), ctypes.byref(h))
         self._h = h.value

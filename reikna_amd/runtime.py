@@ -445,6 +445,26 @@ class _ForeignBuffer:
         self.owner = owner
 
 
+def require_dense(name: str, arr: Any, need_bytes: int) -> None:
+    """Native entry points that take a bare data pointer (the batch kernel,
+    the request server) read and write `need_bytes` consecutive bytes from it:
+    the array must be C-contiguous over its dimensions longer than one, and
+    its buffer must hold that many bytes from the first element on.  Strided
+    or broadcast views are refused here, before any launch."""
+    if not isinstance(arr, Array):
+        raise TypeError(f"{name}: expected a device Array, got {type(arr)}")
+    meta = arr.metadata
+    dense = _c_strides(meta.shape, meta.dtype.itemsize)
+    if any(n > 1 and s != d for n, s, d in zip(meta.shape, meta.strides, dense)):
+        raise ValueError(f"{name}: array strides {meta.strides} are not C-contiguous for shape {meta.shape}")
+    avail = getattr(arr.buffer, "size", None)
+    if avail is None:
+        avail = meta.buffer_size
+    avail -= meta.first_element_offset
+    if avail < need_bytes or meta.nbytes < need_bytes:
+        raise ValueError(f"{name}: {min(avail, meta.nbytes)} bytes from the first element, {need_bytes} needed")
+
+
 class Array(AsArrayMetadata):
     """A device array: a buffer plus ArrayMetadata (grunnur's Array)."""
 

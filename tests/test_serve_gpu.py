@@ -12,6 +12,7 @@ import numpy
 import pytest
 
 import oracle
+from reikna_amd import _lib
 from reikna_amd._lib import BudgetError
 from reikna_amd.fft.serve import FFTServer, RequestSpec, make_specs, run_stream, sample_shapes
 
@@ -184,4 +185,56 @@ def test_failed_waits_are_reported(queue):
     # the native replay raises on a failed launch
     with pytest.raises(RuntimeError, match="gave up"):
         run_stream(server, specs, rate=1000.0)
+    server.close()
+
+
+def test_failed_slot_allocation_is_reported(queue):
+    """A slot whose buffer cannot grow (an allocation failure injected from
+    the host, before any launch) fails its launch cleanly: progress() raises,
+    the admitted requests complete with status 2, their memory is released,
+    and the server serves the next requests correctly."""
+    specs = make_specs(queue, [((4096,), 4), ((2048,), 8)], seed=42)
+    server = FFTServer(queue.device, budget_bytes=64 * 2**20, slots=1,
+                       fault_inject=_lib.RK_SERVER_FAULT_SLOT_ALLOC)
+    server.submit([(i, s, 0) for i, s in enumerate(specs)])
+    with pytest.raises(Exception, match="injected"):
+        server.progress()
+    comps = server.progress()
+    assert sorted(c.id for c in comps) == [0, 1] and all(c.status == 2 for c in comps)
+    assert server.stats["failed_launches"] == 1
+    for c in comps:
+        server.release(c.id)
+    assert server.stats["budget_used"] == 0
+    server.submit([(i + 10, s, 0) for i, s in enumerate(specs)])
+    done = []
+    while len(done) < 2:
+        done += server.progress()
+    for c in done:
+        assert c.status == 0
+        check_output(server, queue, c, specs[c.id - 10], {})
+        server.release(c.id)
+    server.close()
+
+
+def test_submit_is_all_or_nothing(queue):
+    """A batch with a bad request (an id given twice, or one still in use) is
+    refused whole: none of its requests is queued."""
+    specs = make_specs(queue, [((1024,), 2), ((256,), 1)], seed=43)
+    server = FFTServer(queue.device, budget_bytes=64 * 2**20)
+    with pytest.raises(Exception, match="twice"):
+        server.submit([(0, specs[0], 0), (1, specs[1], 0), (0, specs[1], 0)])
+    assert server.stats["pending"] == 0
+    server.submit([(5, specs[0], 0)])
+    with pytest.raises(Exception, match="still in use"):
+        server.submit([(6, specs[1], 0), (5, specs[1], 0)])
+    assert server.stats["pending"] + server.stats["in_flight_requests"] + server.stats["completed"] == 1
+    # ids 0, 1, 6 were never queued: they are free
+    server.submit([(0, specs[0], 0), (1, specs[1], 0), (6, specs[1], 0)])
+    done = []
+    while len(done) < 4:
+        done += server.progress()
+    for c in done:
+        spec = specs[0] if c.id in (0, 5) else specs[1]
+        check_output(server, queue, c, spec, {})
+        server.release(c.id)
     server.close()
