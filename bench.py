@@ -44,7 +44,7 @@ WORKLOADS = {
            "(2^k, 3*2^k, 5*2^k, primes; seed 5), fused 1/N, one launch"),
     # request serving: handled by bench_serve
     "c5": (None, None, "normal", 8, "Poisson stream of c64 FFT requests: 1-D 2^8..2^20 and 2-D 128^2..2048^2, "
-           "batch 1..64 (log-uniform, <= 2^25 elements), arrivals seed 7"),
+           "batch 1..64 (log-uniform), arrivals seed 7"),
     # the computations beside the FFT (transpose, real-input transforms, shift, spectrogram)
     "transpose": ((8192, 8192), None, "normal", 11, "c64 Transpose 8192 x 8192"),
     "rfft": ((64, 2**20), (1,), "normal", 12, "f32 RFFT 64 x 2^20 over axis 1"),

@@ -15,14 +15,27 @@ import numpy
 # ---------------------------------------------------------------------------
 # the reference's oracle: numpy.fft in FP64 (tests/test_fft/test_fft.py:163,169)
 # ---------------------------------------------------------------------------
-def fftn(x: numpy.ndarray, axes: Sequence[int] | None = None) -> numpy.ndarray:
-    """Forward FFT over `axes` in complex128 (numpy.fft.fftn)."""
-    return numpy.fft.fftn(numpy.asarray(x, numpy.complex128), axes=axes)
+def fftn(x: numpy.ndarray, axes: Sequence[int] | None = None, workers: int | None = None) -> numpy.ndarray:
+    """Forward FFT over `axes` in complex128 (numpy.fft.fftn).  With
+    `workers`, the same pocketfft transform through scipy.fft on that many
+    threads (for checks of arrays of 10^8+ elements; pinned against the
+    numpy path in tests/test_oracle.py)."""
+    x = numpy.asarray(x, numpy.complex128)
+    if workers:
+        import scipy.fft
+
+        return scipy.fft.fftn(x, axes=axes, workers=workers)
+    return numpy.fft.fftn(x, axes=axes)
 
 
-def ifftn(x: numpy.ndarray, axes: Sequence[int] | None = None) -> numpy.ndarray:
+def ifftn(x: numpy.ndarray, axes: Sequence[int] | None = None, workers: int | None = None) -> numpy.ndarray:
     """Inverse FFT over `axes` in complex128, normalised by 1/prod(N) (numpy.fft.ifftn)."""
-    return numpy.fft.ifftn(numpy.asarray(x, numpy.complex128), axes=axes)
+    x = numpy.asarray(x, numpy.complex128)
+    if workers:
+        import scipy.fft
+
+        return scipy.fft.ifftn(x, axes=axes, workers=workers)
+    return numpy.fft.ifftn(x, axes=axes)
 
 
 def dft_direct(x: numpy.ndarray, inverse: bool = False) -> numpy.ndarray:

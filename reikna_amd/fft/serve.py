@@ -167,11 +167,13 @@ def sample_shapes(
     one_d: tuple[int, int] = (8, 20),
     two_d: tuple[int, int] = (7, 11),
     max_count: int = 64,
-    max_elements: int = 2**25,
+    max_elements: int | None = None,
 ) -> list[tuple[tuple, int]]:
     """(shape, count) of `n` requests: half 1-D (n = 2^k, k uniform in
     one_d), half 2-D (n x n, n = 2^k, k uniform in two_d); count log-uniform
-    in [1, max_count], capped so a request holds at most max_elements."""
+    in [1, max_count].  With `max_elements`, the count is capped so a request
+    holds at most that many elements (no cap by default: the largest request
+    of the default mix is 64 x 2048^2 = 2^28 elements, 2 GiB)."""
     rng = numpy.random.default_rng(seed)
     out = []
     for _ in range(n):
@@ -181,7 +183,9 @@ def sample_shapes(
             m = 2 ** int(rng.integers(two_d[0], two_d[1] + 1))
             shape = (m, m)
         count = int(numpy.exp(rng.uniform(0.0, numpy.log(max_count + 1))))
-        count = max(1, min(max_count, count, max_elements // math.prod(shape)))
+        count = max(1, min(max_count, count))
+        if max_elements is not None:
+            count = max(1, min(count, max_elements // math.prod(shape)))
         out.append((shape, count))
     return out
 

@@ -274,3 +274,38 @@ def test_wait_give_up_is_reported_and_cleared(queue, monkeypatch):
     # a plan with the default limit is unaffected
     res, _ = run_batch(queue, items, data, data.size)
     check_items(items, data, res)
+
+
+def test_stated_512_uneven_batch(queue):
+    """The batched workload exactly as stated: 512 transforms, lengths
+    log-uniform in [64, 65536] over 2^k, 3*2^k, 5*2^k and primes (seed 5),
+    U[-1, 1) data (seed 6), fused 1/N, all in one launch; every transform
+    against numpy.fft in FP64."""
+    lengths = sample_lengths(512, 64, 65536, seed=5)
+    assert len(lengths) == 512 and min(lengths) >= 64 and max(lengths) <= 65536
+    items = packed_items(lengths, scale=True)
+    data = oracle.test_array((sum(lengths),), seed=6)
+    res, batch = run_batch(queue, items, data, data.size)
+    assert batch.info["launches"] == 1
+    worst = check_items(items, data, res)
+    assert worst <= 1.0
+
+
+def test_stated_512_uneven_batch_r2c(queue):
+    """The same 512 lengths with the real-to-complex packing: every even
+    length an r2c item (numpy.fft.rfft of its real input), the odd ones
+    complex, fused 1/N, one launch; every transform checked."""
+    lengths = sample_lengths(512, 64, 65536, seed=5)
+    kinds = [n % 2 == 0 for n in lengths]
+    items = packed_items(lengths, scale=True, r2c=kinds)
+    flat = r2c_batch_data(items, seed=6)
+    batch = BatchFFT(queue.device, items)
+    inp = Array.from_host(queue, flat)
+    n_out = max(it.out_offset + (it.count - 1) * it.out_dist + it.out_length for it in items)
+    out = Array.from_host(queue, numpy.zeros(n_out, numpy.complex64))
+    batch(queue, out, inp)
+    res = out.get(queue)
+    batch.check()
+    assert batch.info["launches"] == 1
+    assert sum(kinds) > 300 and sum(not k for k in kinds) > 50
+    check_r2c_items(items, flat, res)

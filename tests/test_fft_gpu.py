@@ -211,6 +211,43 @@ def test_fused_transformations(queue):
     assert oracle.rel_l2(out.get(queue), ref) <= oracle.tolerance(shape, axes)
 
 
+@pytest.mark.parametrize(
+    "shape,axes,rows",
+    [
+        ((4096, 997), (1,), None),              # single-kernel Bluestein (chirp pre-op + fused input)
+        ((8, 65521), (1,), None),               # Bluestein as two multi-pass chains
+        ((64, 2**20), (1,), [0, 17, 40, 63]),   # the three-pass streaming chain
+        ((256, 256, 256), (0, 1, 2), None),     # three axes, the input fused into the first axis's pass
+    ],
+    ids=["4096x997", "8x65521", "64x2^20", "256^3"],
+)
+@pytest.mark.parametrize("inverse", [False, True])
+def test_fused_input_on_every_plan_kind(queue, shape, axes, rows, inverse):
+    """An input transformation (mul_param) and an output one (div_const)
+    fused on the plans where the first pass has its own pre-op or is one of
+    several: reikna fuses its load/store functions with pad, chirp and
+    kweights (fft.mako:867-890, tests/test_fft/test_fft.py:173-193)."""
+    data = oracle.test_array(shape, seed=31)
+    dev = Array.from_host(queue, data)
+    out = Array.empty_like(queue.device, dev)
+    fft = FFT(dev, axes=axes)
+    scale = mul_param(dev, numpy.float32)
+    fft.parameter.input.connect(scale, scale.output, input_prime=scale.input, param=scale.param)
+    div = div_const(dev, 4.0)
+    fft.parameter.output.connect(div, div.input, output_prime=div.output)
+    fftc = fft.compile(queue.device)
+    fftc(queue, out, dev, -1.5, int(inverse))
+    res = out.get(queue)
+    x = data * numpy.float32(-1.5)
+    trf = oracle.ifftn if inverse else oracle.fftn
+    tol = oracle.tolerance(shape, axes)
+    if rows is None:
+        err = oracle.rel_l2(res, trf(x, axes) / 4.0)
+    else:
+        err = max(oracle.rel_l2(res[r], trf(x[r], [a - 1 for a in axes]) / 4.0) for r in rows)
+    assert err <= tol, f"{shape} over {axes}: rel-L2 {err:.3g} > {tol:.3g}"
+
+
 def test_strided_input_and_output(queue):
     """Non-contiguous arrays: reikna honours arbitrary strides (transformation_modules.py:32-69)."""
     base = oracle.test_array((64, 2048), seed=9)

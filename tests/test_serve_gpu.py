@@ -238,3 +238,38 @@ def test_submit_is_all_or_nothing(queue):
         check_output(server, queue, c, spec, {})
         server.release(c.id)
     server.close()
+
+
+@pytest.mark.timeout(600)
+def test_stated_serving_mix(queue):
+    """The serving workload as stated: 1-D 256..2^20 and 2-D 128^2..2048^2
+    transforms, batch 1..64 (no cap: the largest request is 64 x 2048^2),
+    N(0, 1) inputs, arrivals at one Poisson rate; every completion checked
+    against numpy.fft in FP64 (threaded pocketfft for these sizes)."""
+    import os
+
+    shapes = sample_shapes(24, seed=7)
+    assert max(c * math.prod(s) for s, c in shapes) > 2**25   # requests beyond the old cap
+    specs = make_specs(queue, shapes, seed=8)
+    server = FFTServer(queue.device, budget_bytes=16 * GiB, slots=4, plain_min_elements=2**22)
+    keep = set(range(len(specs)))
+    res = run_stream(server, specs, rate=2000.0, seed=7, keep=keep)
+    assert res.completed == len(specs) and 0 < res.p50_ms <= res.p99_ms
+    workers = min(16, os.cpu_count() or 1)
+    hosts = {}
+    for c in sorted(res.completions, key=lambda c: c.id):
+        spec = specs[c.id]
+        assert c.status == 0
+        key = id(spec.input)
+        if key not in hosts:
+            hosts[key] = spec.input.get(queue)
+        x = hosts[key][: spec.elements].reshape(spec.count, *spec.shape)
+        ref = oracle.fftn(x, tuple(range(1, 1 + len(spec.shape))), workers=workers)
+        out = server.output(c, spec).get(queue)
+        server.release(c.id)
+        err = oracle.rel_l2(out, ref)
+        tol = oracle.tolerance(spec.shape)
+        assert err <= tol, f"request {c.id} {spec.shape} x {spec.count}: rel-L2 {err:.3g} > {tol:.3g}"
+        del ref, out
+    assert server.stats["budget_used"] == 0
+    server.close()
