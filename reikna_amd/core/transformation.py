@@ -176,7 +176,7 @@ class TransformationTree:
             old = self.root_parameters.get(param.name)
             if old is not None and old != param:
                 # the same array passed as 'i' and 'o' of a nested computation: merge into 'io'
-                if old.annotation._as_type() != param.annotation._as_type():
+                if old.annotation.type != param.annotation.type:
                     raise ValueError(f"Type mismatch for two parameters with name `{param.name}`")
                 if old.default != param.default:
                     raise ValueError(f"Default value mismatch for two parameters with name `{param.name}`")
@@ -252,7 +252,7 @@ class TransformationTree:
                 continue
             node_ann = self.leaf_parameters[node_name].annotation
             tr_ann = trf.signature.parameters[tr_name].annotation
-            if tr_ann._as_type() != node_ann._as_type():
+            if tr_ann.type != node_ann.type:
                 raise ValueError(
                     "Incompatible types of the transformation parameter "
                     f"'{tr_name}' ({tr_ann}) and the node '{node_name}' ({node_ann})"

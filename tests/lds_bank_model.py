@@ -10,7 +10,7 @@ either compute mapping, and prices each instruction by its worst bank:
 * 64-bit accesses: two half-waves, each over 32 slots of two banks;
 * 32-bit accesses (tiles split into re/im halves): 64 lanes over 64 banks.
 
-``python -m reikna_amd.csrc.lds_bank_model`` prints, for every catalogue tile
+``python tests/lds_bank_model.py`` prints, for every catalogue tile
 whose pitch the rule changes, the model's cycles per ideal cycle at the base
 (odd) pitch and at the rule's pitch, averaged over the load/store modes and
 both mappings.
@@ -18,10 +18,13 @@ both mappings.
 
 from __future__ import annotations
 
-try:
-    from .build import catalog_entries, lds_pitch, radix_list, tile_pitch
-except ImportError:  # run as a script from this directory
-    from build import catalog_entries, lds_pitch, radix_list, tile_pitch
+import sys
+from pathlib import Path
+
+if __package__ in (None, ""):   # run as a script: the repository root holds the package
+    sys.path.insert(0, str(Path(__file__).resolve().parent.parent))
+
+from reikna_amd.csrc.build import catalog_entries, lds_pitch, radix_list, tile_pitch  # noqa: E402
 
 # (column mapping, load walk, store walk) combinations the planner produces
 MODES = [

@@ -133,7 +133,32 @@ def test_pure_parallel_from_transformation_compiles(vdev):
     pp = PureParallel.from_trf(trf, trf.input)
     c = pp.compile(vdev)
     assert len(c.kernel_sources) == 1
-    assert "rk_skip" in c.kernel_sources[0]
+    src = c.kernel_sources[0]
+    # one thread per element over a flat grid, the index decoded with literal extents
+    assert "rk_elementwise" in src and "rk_flat >= 3000ull" in src and "% 30ull" in src
+
+
+def test_pure_parallel_snippet_and_errors(vdev):
+    import pytest
+
+    from reikna_amd.core import Parameter, Annotation
+
+    arr = ArrayMetadata((7, 5, 3), numpy.float32)
+    pp = PureParallel(
+        [Parameter("out", Annotation(arr, "o")), Parameter("inp", Annotation(arr, "i"))],
+        "${out.store_idx}(${idxs[0]}, ${idxs[1]}, ${idxs[2]}, ${inp.load_idx}(${idxs[0]}, ${idxs[1]}, ${idxs[2]}) * 2);",
+    )
+    src = pp.compile(vdev).kernel_sources[0]
+    assert "rk_flat >= 105ull" in src and "% 3ull" in src and "% 5ull" in src
+    trf = copy(arr)
+    with pytest.raises(ValueError, match="cannot be a scalar"):
+        from reikna_amd.transformations import mul_param
+
+        m = mul_param(arr, numpy.float32)
+        PureParallel.from_trf(m, m.param)
+    other = copy(arr)
+    with pytest.raises(ValueError, match="must belong"):
+        PureParallel.from_trf(trf, other.input)
 
 
 def test_plan_streaming_accesses(vdev):

@@ -5,7 +5,7 @@ sizes against those the device compiler reports for the same tiles."""
 import pytest
 
 from reikna_amd.csrc.build import lds_pitch, tile_lds_bytes, tile_pitch
-from reikna_amd.csrc.lds_bank_model import score, tile_cycles
+from tests.lds_bank_model import score, tile_cycles
 
 
 @pytest.mark.parametrize("length,c", [(1024, 4), (1024, 16), (512, 8), (256, 16), (2048, 8)])

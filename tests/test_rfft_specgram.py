@@ -30,6 +30,37 @@ def specgram_reference(x, fft_size, noverlap, pad_to):
     return spec.T
 
 
+@pytest.mark.parametrize("params", [dict(fft_size=256, noverlap=128, pad_to=None),
+                                    dict(fft_size=1024, noverlap=900, pad_to=2048),
+                                    dict(fft_size=100, noverlap=0, pad_to=128)])
+def test_specgram_reference_against_matplotlib(params):
+    """The FP64 restatement above is matplotlib's spectrogram (the reference
+    demo's own comparison, demo_specgram.py:59,306): mlab.specgram in
+    magnitude mode divides |FFT| by the window sum; detrending off, the
+    symmetric Hann window, one-sided spectrum.  A planted mistake (the
+    periodic Hann window) is caught."""
+    mlab = pytest.importorskip("matplotlib.mlab")
+    rng = numpy.random.default_rng(12)
+    x = rng.standard_normal(20000)
+    spec, _, _ = mlab.specgram(x, NFFT=params["fft_size"], Fs=2.0, noverlap=params["noverlap"],
+                               pad_to=params["pad_to"], mode="magnitude")
+    win_sum = numpy.hanning(params["fft_size"]).sum()
+    ref = specgram_reference(x, **params)
+    assert ref.shape == spec.shape
+    assert numpy.allclose(ref, spec * win_sum, rtol=1e-10, atol=1e-10 * numpy.abs(ref).max())
+    # planted mistake: the periodic window 0.5 - 0.5 cos(2 pi n / N) differs
+    n = numpy.arange(params["fft_size"])
+    periodic = 0.5 - 0.5 * numpy.cos(2 * numpy.pi * n / params["fft_size"])
+    assert not numpy.allclose(periodic, numpy.hanning(params["fft_size"]))
+
+
+def test_hann_window_is_numpy_hanning():
+    from reikna_amd.fft.spectrogram import hann_window
+
+    for n in (1, 2, 7, 256, 1000):
+        assert numpy.allclose(hann_window(n), numpy.hanning(n), rtol=0, atol=1e-15)
+
+
 def test_errors():
     with pytest.raises(ValueError):
         RFFT(ArrayMetadata((15,), numpy.float32))
@@ -113,3 +144,26 @@ def test_spectrogram_gpu(queue, params):
     # the reference demo's check (demo_specgram.py:251)
     assert numpy.allclose(res, ref, atol=1e-4 * params["fft_size"], rtol=1e-4)
     assert numpy.linalg.norm(res - ref) / numpy.linalg.norm(ref) <= oracle.tolerance((res.shape[0] * 2,))
+
+
+@pytest.mark.gpu
+def test_spectrogram_custom_window_gpu(queue):
+    """A window other than Hann (a Blackman window as an array, and a
+    function of the length) reaches the fused input transformation."""
+    rng = numpy.random.default_rng(4)
+    x = rng.standard_normal(30000).astype(numpy.float32)
+    x_dev = Array.from_host(queue, x)
+    for window in (numpy.blackman(512), numpy.bartlett):
+        sp = Spectrogram(x_dev, fft_size=512, noverlap=384, pad_to=1024, window=window)
+        out = Array.empty_like(queue.device, sp.parameter.output)
+        sp.compile(queue.device)(queue, out, x_dev)
+        w = window if isinstance(window, numpy.ndarray) else window(512)
+        step = 512 - 384
+        frames = (x.size - 384) // step
+        rows = numpy.zeros((frames, 1024))
+        for i in range(frames):
+            rows[i, :512] = x[i * step: i * step + 512].astype(numpy.float64) * w
+        ref = numpy.abs(numpy.fft.fft(rows, axis=1))[:, :513].T
+        res = out.get(queue)
+        assert res.shape == ref.shape
+        assert numpy.linalg.norm(res - ref) / numpy.linalg.norm(ref) <= oracle.tolerance((1024,))
