@@ -209,9 +209,16 @@ typedef struct {
     int64_t in_dims[3];   /* folded (O, axis length read, I) of the pass input */
     int stream;           /* 1 if the kernel loads and stores with the nontemporal hint */
     int fixed_modes;      /* 1 if the kernel has load_mode/store_mode compiled in */
+    int nt_load;          /* 1 if the kernel loads with the nontemporal hint */
+    int nt_store;         /* 1 if the kernel stores with the nontemporal hint */
+    int wave;             /* 1 if each wave holds whole instances and the exchanges
+                             between stages stay inside it (row mapping) */
 } rk_fft_pass_info;
 
 int rk_fft_plan_num_passes(const rk_fft_plan* plan, int* n);
+/* Chunked execution: every pass runs over *chunk_outer outer-batch indices
+   before the next chunk starts (of *n_outer); 0 when the plan is not chunked. */
+int rk_fft_plan_chunking(const rk_fft_plan* plan, int64_t* chunk_outer, int64_t* n_outer);
 int rk_fft_plan_pass_info(const rk_fft_plan* plan, int pass, rk_fft_pass_info* info);
 
 /* Replace the kernel of one pass with a run-time compiled one whose

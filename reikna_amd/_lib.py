@@ -154,6 +154,9 @@ class FftPassInfo(ctypes.Structure):
         ("in_dims", ctypes.c_int64 * 3),
         ("stream", ctypes.c_int),
         ("fixed_modes", ctypes.c_int),
+        ("nt_load", ctypes.c_int),
+        ("nt_store", ctypes.c_int),
+        ("wave", ctypes.c_int),
     ]
 
 
@@ -245,6 +248,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "rk_fft_execute_fused": (_I, [_P, _P, _P, _I, _P, _P, _P, _SZ, _P, _SZ]),
     "rk_fft_plan_destroy": (_I, [_P]),
     "rk_fft_plan_num_passes": (_I, [_P, ctypes.POINTER(_I)]),
+    "rk_fft_plan_chunking": (_I, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "rk_fft_plan_pass_info": (_I, [_P, _I, ctypes.POINTER(FftPassInfo)]),
     "rk_fft_plan_set_pass_kernel": (_I, [_P, _I, _P]),
     "rk_fft_plan_set_timing": (_I, [_P, _I]),

@@ -137,6 +137,8 @@ BLUE_SINGLE_MAX = 8192               # longest single-kernel Bluestein convoluti
 VAR_MODE_DD = 1 << 8                 # load/store modes DIRECT/DIRECT compiled in (RK_VAR_MODE_DD)
 VAR_MODE_DF = 1 << 9                 # load/store modes DIRECT/FLAT compiled in (RK_VAR_MODE_DF)
 VAR_MODE_FF = 1 << 10                # load/store modes FLAT/FLAT compiled in (RK_VAR_MODE_FF)
+VAR_NT_LOAD = 1 << 11                # nontemporal loads only (RK_VAR_NT_LOAD)
+VAR_NT_STORE = 1 << 12               # nontemporal stores only (RK_VAR_NT_STORE)
 FACTOR_MAX = 256                     # longest pass of a multi-pass chain (rk_fft_plan.cpp factor_max)
 
 
@@ -154,6 +156,16 @@ def catalog_entries() -> list[tuple[int, int, int]]:
     for length, c, v in list(ents):
         if length <= STREAM_MAX and v in (VAR_PLAIN, VAR_TWIDDLE):
             ents.add((length, c, v | VAR_STREAM))
+    # chains run in chunks through the Infinity Cache (rk_fft_plan.cpp): the
+    # first pass of a chain loads the caller's input with the nontemporal
+    # hint and stores plainly, the last pass loads plainly and stores the
+    # caller's output with the hint
+    for length, c, v in list(ents):
+        if length <= FACTOR_MAX and c == tile_c_col(length):
+            if v == VAR_PLAIN:
+                ents.add((length, c, v | VAR_NT_LOAD))
+            elif v == VAR_TWIDDLE:
+                ents.add((length, c, v | VAR_NT_STORE))
     # the column passes of large multi-pass transforms with their modes
     # compiled in: the first pass loads columns and stores the tile flat (or
     # in columns), the later passes load and store columns.  Only lengths
@@ -163,10 +175,11 @@ def catalog_entries() -> list[tuple[int, int, int]]:
     # measured 1-2 % slower with them.
     # The same for the plain (non-streaming) column passes of such lengths:
     # strided axes of multi-axis transforms and chains below the cache size.
+    nt = VAR_STREAM | VAR_NT_LOAD | VAR_NT_STORE
     for length, c, v in list(ents):
         pow2 = length & (length - 1) == 0
-        if not pow2 and length <= FACTOR_MAX and c == tile_c_col(length) and v & ~VAR_STREAM in (VAR_PLAIN, VAR_TWIDDLE):
-            if v & ~VAR_STREAM == VAR_PLAIN:
+        if not pow2 and length <= FACTOR_MAX and c == tile_c_col(length) and v & ~nt in (VAR_PLAIN, VAR_TWIDDLE):
+            if v & ~nt == VAR_PLAIN:
                 ents.add((length, c, v | VAR_MODE_DF))
                 ents.add((length, c, v | VAR_MODE_DD))
             else:
@@ -261,7 +274,7 @@ def generate_batch_cases(out_dir: Path) -> list[Path]:
 
 
 def variant_cost(v: int) -> float:
-    v &= ~(VAR_STREAM | VAR_MODE_DD | VAR_MODE_DF | VAR_MODE_FF)
+    v &= ~(VAR_STREAM | VAR_NT_LOAD | VAR_NT_STORE | VAR_MODE_DD | VAR_MODE_DF | VAR_MODE_FF)
     return 2.0 if v & (1 << 4) else 1.3 if v else 1.0
 
 
@@ -282,17 +295,9 @@ def generate_catalog(out_dir: Path) -> list[Path]:
             '#include "rk_internal.hpp"',
             f"namespace rk_catalog_{k} {{",
             "template <int L, int C, int V>",
-            "__global__ void __launch_bounds__((rkfft::pass_threads<L, C>()),",
-            "    ((V & RK_VAR_MODE_DD) && (V & 16) && rkfft::pass_threads<L, C>() <= 256) ? 3 : 1)",
+            "__global__ void __launch_bounds__((rkfft::pass_threads<L, C>()), (rkfft::pass_min_waves<L, C, V>()))",
             "rk_fft_pass(rk_pass_args a) {",
-            "    constexpr int VB = V & (RK_VAR_STREAM - 1);",
-            "    constexpr int M = (V & RK_VAR_MODE_DD) ? RK_MODES(RK_MODE_DIRECT, RK_MODE_DIRECT)",
-            "                    : (V & RK_VAR_MODE_DF) ? RK_MODES(RK_MODE_DIRECT, RK_MODE_FLAT)",
-            "                    : (V & RK_VAR_MODE_FF) ? RK_MODES(RK_MODE_FLAT, RK_MODE_FLAT) : 0;",
-            "    if constexpr ((V & RK_VAR_STREAM) != 0)",
-            "        rkfft::fft_pass_body<L, C, VB, rkfft::StreamIO, M>(a, rkfft::StreamIO{});",
-            "    else",
-            "        rkfft::fft_pass_body<L, C, VB, rkfft::PlainIO, M>(a, rkfft::PlainIO{});",
+            "    rkfft::fft_pass_entry<L, C, V>(a);",
             "}",
             "static const rk::KernelEntry entries[] = {",
         ]

@@ -371,12 +371,15 @@ struct TwiddleWalk {
 // runs at 5.8 TB/s plain and 6.0 TB/s with the hint).  Passes whose data fit
 // in that cache keep the plain accesses: the hint costs the 256^3 transform,
 // whose in-place passes re-read what the previous pass just wrote, 22 %.
+// The hint is chosen per side (NTL loads, NTS stores): a chunked chain whose
+// intermediate stays in that cache between passes streams only its first
+// pass's loads and its last pass's stores.
 typedef float rk_v2f __attribute__((ext_vector_type(2)));
-template <bool NT> struct GlobalIO {
+template <bool NTL, bool NTS = NTL> struct GlobalIO {
     static constexpr bool kBranchFree = true;
     __device__ __forceinline__ float2 load(const rk_pass_args& a, rk_u64 /*o*/, rk_u64 /*m*/, rk_u64 /*i*/,
                                            rk_i64 addr) const {
-        if constexpr (NT) {
+        if constexpr (NTL) {
             const rk_v2f x = __builtin_nontemporal_load(reinterpret_cast<const rk_v2f*>(a.in + addr));
             return make_float2(x.x, x.y);
         } else {
@@ -385,7 +388,7 @@ template <bool NT> struct GlobalIO {
     }
     __device__ __forceinline__ void store(const rk_pass_args& a, rk_u64 /*o*/, rk_u64 /*k*/, rk_u64 /*i*/,
                                           rk_i64 addr, float2 v) const {
-        if constexpr (NT) {
+        if constexpr (NTS) {
             rk_v2f x;
             x.x = v.x;
             x.y = v.y;
@@ -442,6 +445,15 @@ struct CoherentIO {
 #define RK_VAR_MODE_DD 256
 #define RK_VAR_MODE_DF 512
 #define RK_VAR_MODE_FF 1024
+// Bits of a catalogue kernel's variant number: nontemporal loads only / stores
+// only (GlobalIO<true, false> / GlobalIO<false, true>; RK_VAR_STREAM is both).
+#define RK_VAR_NT_LOAD 2048
+#define RK_VAR_NT_STORE 4096
+// Bit of a catalogue kernel's variant number: the row mapping is compiled in
+// and each wave holds whole instances (64 % T == 0), so the exchanges between
+// Stockham stages stay inside one wave and need no work-group barrier; only
+// the staged global sides (COLUMN, FLAT) exchange across waves.
+#define RK_VAR_WAVE 8192
 // MODES template argument of fft_pass_tile: 0 reads the modes from the pass
 // arguments, RK_MODES(load, store) compiles them in.
 #define RK_MODES(load, store) (16 | (int)(load) | ((int)(store) << 2))
@@ -451,37 +463,115 @@ template <int V> struct Variant {
     static constexpr bool BLUE = ((V >> 4) & 1) != 0;
 };
 
-template <int L, int V, class IO>
-__device__ __forceinline__ float2 load_elem(const rk_pass_args& a, const IO& io, const Inst& ic, int p) {
-    constexpr int PRE = Variant<V>::PRE;
-    const rk_u64 m = ic.jpass + (rk_u64)p * a.pos_mult;
-    const bool ok = ic.valid && m < a.n_valid_in;
-    float2 x = make_float2(0.0f, 0.0f);
-    if constexpr (IO::kBranchFree) {
-        // a lane with nothing to load reads its instance's first position (a
-        // valid address, different per instance) and discards it: one shared
-        // clamp address would be a hot spot for loads that bypass the caches
-        const rk_i64 spare = ic.in_base + (rk_i64)(ic.jpass < a.n_valid_in ? ic.jpass : 0) * a.in_sa;
-        const float2 y = io.load(a, ic.o, m, ic.i, ok ? ic.in_base + (rk_i64)m * a.in_sa : spare);
-        x = ok ? y : x;
-    } else {
-        if (ok) x = io.load(a, ic.o, m, ic.i, ic.in_base + (rk_i64)m * a.in_sa);
-    }
-    x.y *= a.in_sign;
-    if constexpr (PRE == RK_VAR_PRE_CHIRP) x = c_mul(x, a.chirp[ok ? m : 0]);
-    return x;   // the inter-pass twiddle (PRE_TWIDDLE) is applied by the caller's TwiddleWalk
+// Padded sides: only a Bluestein chirp pass reads positions beyond its data
+// (the input beyond n_valid_in reads as zero) and only the last pass of a
+// Bluestein chain leaves positions unwritten (beyond n_valid_out); every
+// other pass reads and writes all N positions of its axis (the planners check
+// this), so its elements need no per-element bounds.
+template <int V> __host__ __device__ constexpr bool pad_in() { return Variant<V>::PRE == RK_VAR_PRE_CHIRP; }
+template <int V> __host__ __device__ constexpr bool pad_out() { return Variant<V>::POST == RK_VAR_POST_CONJ_CHIRP; }
+
+// One thread's walk over the positions p0 + c of one instance on one side
+// (c a compile-time offset): axis position and element address advance by
+// uniform steps computed once per thread, so an element costs one add and no
+// multiply.
+struct Walk {
+    rk_u64 m0, mstep;   // axis position of position p0, and per unit of c
+    rk_i64 a0, astep;   // element address of position p0, and per unit of c
+};
+
+__device__ __forceinline__ Walk in_walk(const rk_pass_args& a, const Inst& ic, int p0) {
+    Walk w;
+    w.m0 = ic.jpass + (rk_u64)p0 * a.pos_mult;
+    w.mstep = a.pos_mult;
+    w.a0 = ic.in_base + (rk_i64)w.m0 * a.in_sa;
+    w.astep = (rk_i64)a.pos_mult * a.in_sa;
+    return w;
 }
 
-template <int L, int V, class IO>
-__device__ __forceinline__ void store_elem(const rk_pass_args& a, const IO& io, const Inst& ic, int k, float2 y) {
+// output position k of an instance sits at axis position kbase + k * dim_NS
+__device__ __forceinline__ Walk out_walk(const rk_pass_args& a, const Inst& ic, int k0) {
+    Walk w;
+    w.m0 = ic.kbase + (rk_u64)k0 * a.dim_NS;
+    w.mstep = a.dim_NS;
+    w.a0 = ic.out_base + (rk_i64)w.m0 * a.out_sa;
+    w.astep = (rk_i64)a.dim_NS * a.out_sa;
+    return w;
+}
+
+// Loads position p0 + c of the walk's instance (raw: no sign, no pre-op).  An
+// invalid instance (past n_inst) was decomposed as instance 0, so its
+// addresses are valid: it reads them and its results are never stored.
+template <int V, class IO>
+__device__ __forceinline__ float2 load_at(const rk_pass_args& a, const IO& io, const Inst& ic, const Walk& w, int c) {
+    const rk_u64 m = w.m0 + (rk_u64)c * w.mstep;
+    const rk_i64 addr = w.a0 + (rk_i64)c * w.astep;
+    const float2 zero = make_float2(0.0f, 0.0f);
+    if constexpr (pad_in<V>()) {
+        const bool ok = ic.valid && m < a.n_valid_in;
+        if constexpr (IO::kBranchFree) {
+            // a lane with nothing to load reads its instance's first position
+            // (a valid address, different per instance) and discards it: one
+            // shared clamp address would be a hot spot for loads that bypass
+            // the caches
+            const rk_i64 spare = ic.in_base + (rk_i64)(ic.jpass < a.n_valid_in ? ic.jpass : 0) * a.in_sa;
+            const float2 y = io.load(a, ic.o, m, ic.i, ok ? addr : spare);
+            return ok ? y : zero;
+        } else {
+            return ok ? io.load(a, ic.o, m, ic.i, addr) : zero;
+        }
+    } else {
+        if constexpr (IO::kBranchFree) {
+            return io.load(a, ic.o, m, ic.i, addr);
+        } else {
+            return ic.valid ? io.load(a, ic.o, m, ic.i, addr) : zero;
+        }
+    }
+}
+
+// Input-side pre-processing of n loaded values at positions p0 + c_i
+// (c_i = cpos(i)): conjugation for the inverse direction (a uniform branch),
+// then the Bluestein chirp (compile-time).  The inter-pass twiddle (PRE_TWIDDLE)
+// is applied by the caller's TwiddleWalk.
+template <int V, int NV, class FC>
+__device__ __forceinline__ void pre_ops(const rk_pass_args& a, const Inst& ic, const Walk& w, float2* x, FC cpos) {
+    if (a.in_sign < 0.0f) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) x[i].y = -x[i].y;
+    }
+    if constexpr (Variant<V>::PRE == RK_VAR_PRE_CHIRP) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const rk_u64 m = w.m0 + (rk_u64)cpos(i) * w.mstep;
+            const bool ok = ic.valid && m < a.n_valid_in;
+            x[i] = c_mul(x[i], a.chirp[ok ? m : 0]);
+        }
+    }
+}
+
+// The output-side operations of one value at axis position kabs: the
+// compile-time post-op, then conjugation and scale for the direction.
+template <int V>
+__device__ __forceinline__ float2 post_op(const rk_pass_args& a, rk_u64 kabs, float2 y, bool adjust) {
     constexpr int POST = Variant<V>::POST;
-    const rk_u64 kabs = ic.kbase + (rk_u64)k * a.dim_NS;
-    if (!ic.valid || kabs >= a.n_valid_out) return;
     if constexpr (POST == RK_VAR_POST_KMUL_CONJ) y = c_conj(c_mul(y, a.bspec[kabs]));
     if constexpr (POST == RK_VAR_POST_CONJ_CHIRP) y = c_mul(c_conj(y), a.chirp[kabs]);
-    y.y *= a.out_sign;
-    y = c_scale(y, a.out_scale);
-    io.store(a, ic.o, kabs, ic.i, ic.out_base + (rk_i64)kabs * a.out_sa, y);
+    if (adjust) {
+        y.y *= a.out_sign;
+        y = c_scale(y, a.out_scale);
+    }
+    return y;
+}
+
+// Stores position k0 + c of the walk's instance (the caller skips invalid
+// instances); a padded side skips positions at or beyond n_valid_out.
+template <int V, class IO>
+__device__ __forceinline__ void store_at(const rk_pass_args& a, const IO& io, const Inst& ic, const Walk& w, int c,
+                                         float2 y, bool adjust) {
+    const rk_u64 kabs = w.m0 + (rk_u64)c * w.mstep;
+    if constexpr (pad_out<V>())
+        if (kabs >= a.n_valid_out) return;
+    io.store(a, ic.o, kabs, ic.i, w.a0 + (rk_i64)c * w.astep, post_op<V>(a, kabs, y, adjust));
 }
 
 // ------------------------------------------------------------------------
@@ -492,17 +582,35 @@ __device__ __forceinline__ void store_elem(const rk_pass_args& a, const IO& io, 
 // real and imaginary parts travel separately through a buffer of floats, so
 // the tile needs half the LDS (two work-groups then fit on a CU); `w` and `r`
 // may be the same array.
-template <bool SPLIT, int NW, int NR, class FW, class FR>
+// Ordering point of an exchange.  Work-group scope: a barrier with
+// work-group fences (__syncthreads).  Wave scope (every position a lane
+// writes or reads belongs to instances its own wave holds): no barrier; the
+// wave-scope fences keep the compiler from moving LDS accesses across the
+// point, and the wait for the wave's outstanding LDS operations
+// (lgkmcnt(0): s_waitcnt encoding 0xc07f leaves vmcnt and expcnt at their
+// maxima) completes its writes before its reads.
+template <bool WAVE> __device__ __forceinline__ void exchange_sync() {
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_s_waitcnt(0xc07f);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
+template <bool SPLIT, int NW, int NR, bool WAVE = false, class FW, class FR>
 __device__ __forceinline__ void lds_exchange(void* base, const float2* w, float2* r, FW wpos, FR rpos) {
     if constexpr (!SPLIT) {
         float2* lds = reinterpret_cast<float2*>(base);
-        __syncthreads();
+        exchange_sync<WAVE>();
 #pragma unroll
         for (int i = 0; i < NW; ++i) {
             const int p = wpos(i);
             if (p >= 0) lds[p] = w[i];
         }
-        __syncthreads();
+        exchange_sync<WAVE>();
 #pragma unroll
         for (int i = 0; i < NR; ++i) {
             const int p = rpos(i);
@@ -513,25 +621,25 @@ __device__ __forceinline__ void lds_exchange(void* base, const float2* w, float2
         float wy[NW], rx[NR];
 #pragma unroll
         for (int i = 0; i < NW; ++i) wy[i] = w[i].y;
-        __syncthreads();
+        exchange_sync<WAVE>();
 #pragma unroll
         for (int i = 0; i < NW; ++i) {
             const int p = wpos(i);
             if (p >= 0) f[p] = w[i].x;
         }
-        __syncthreads();
+        exchange_sync<WAVE>();
 #pragma unroll
         for (int i = 0; i < NR; ++i) {
             const int p = rpos(i);
             rx[i] = p >= 0 ? f[p] : 0.0f;
         }
-        __syncthreads();
+        exchange_sync<WAVE>();
 #pragma unroll
         for (int i = 0; i < NW; ++i) {
             const int p = wpos(i);
             if (p >= 0) f[p] = wy[i];
         }
-        __syncthreads();
+        exchange_sync<WAVE>();
 #pragma unroll
         for (int i = 0; i < NR; ++i) {
             const int p = rpos(i);
@@ -572,7 +680,7 @@ __device__ __forceinline__ int stage_out_pos(int t, int i) {  // output position
     return (j / NS) * NS * R + (j % NS) + r * NS;
 }
 
-template <int L, int S, bool SPLIT, int VM>
+template <int L, int S, bool SPLIT, int VM, bool WAVE = false>
 __device__ __forceinline__ void run_stages(float2 (&v)[VM], void* my, int t, const float2* twl) {
     using P = Plan<L>;
     constexpr int R = P::radix(S);
@@ -598,11 +706,11 @@ __device__ __forceinline__ void run_stages(float2 (&v)[VM], void* my, int t, con
     if constexpr (S + 1 < P::NSTAGE) {
         constexpr int R1 = P::radix(S + 1);
         constexpr int Q1 = P::q(S + 1);
-        lds_exchange<SPLIT, Q * R, Q1 * R1>(
+        lds_exchange<SPLIT, Q * R, Q1 * R1, WAVE>(
             my, v, v,
             [&](int i) { return stage_valid<L, S>(t, i / R) ? lds_pad(stage_out_pos<L, S>(t, i)) : -1; },
             [&](int i) { return stage_valid<L, S + 1>(t, i / R1) ? lds_pad(stage_in_pos<L, S + 1>(t, i)) : -1; });
-        run_stages<L, S + 1, SPLIT, VM>(v, my, t, twl);
+        run_stages<L, S + 1, SPLIT, VM, WAVE>(v, my, t, twl);
     }
 }
 
@@ -637,9 +745,10 @@ template <int L, int C> __host__ __device__ constexpr int tile_lds_bytes() {
 // (a, tile, tid) alone, so a caller with more threads than C*T may give the
 // surplus threads tid = C*T - 1: they repeat that thread's work, writing the
 // same values to the same addresses, and take part in every barrier.
-template <int L, int C, int V, class IO, int MODES = 0>
+template <int L, int C, int V, class IO, int MODES = 0, bool WAVE = false>
 __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& io, float2* lds, rk_u64 tile, int tid) {
     using P = Plan<L>;
+    static_assert(!WAVE || (64 % P::T == 0), "wave-local stages need whole instances per wave");
     constexpr int T = P::T;
     constexpr int NT = C * T;
     constexpr int R0 = P::R0;
@@ -655,8 +764,9 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
     const rk_u32 load_mode = MODES ? (rk_u32)(MODES & 3) : a.load_mode;
     const rk_u32 store_mode = MODES ? (rk_u32)((MODES >> 2) & 3) : a.store_mode;
 
-    // compute mapping (see fft_args.h): lanes along an instance or across instances
-    const bool colmap = a.cmap != 0;
+    // compute mapping (see fft_args.h): lanes along an instance or across
+    // instances (WAVE: the row mapping, compiled in)
+    const bool colmap = WAVE ? false : a.cmap != 0;
     const int c = colmap ? (tid % C) : (tid / T);
     const int t = colmap ? (tid / C) : (tid % T);
     const int cbase = c * LP;
@@ -671,8 +781,10 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
     const Inst ic = decompose<L>(a, inst0 + c);
     auto own_pos = [&](int i) { return cbase + lds_pad(stage_in_pos<L, 0>(t, i)); };
     if (load_mode == RK_MODE_DIRECT) {
+        const Walk w = in_walk(a, ic, t);
 #pragma unroll
-        for (int r = 0; r < R0; ++r) v[r] = load_elem<L, V, IO>(a, io, ic, t + r * T);
+        for (int r = 0; r < R0; ++r) v[r] = load_at<V, IO>(a, io, ic, w, r * T);
+        pre_ops<V, R0>(a, ic, w, v, [](int r) { return r * T; });
         if constexpr (TWID) {
             TwiddleWalk tw;
             tw.init(a, ic.l, t, T);
@@ -694,8 +806,11 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
             } else {
                 if (ok) x = io.load(a, inst0 + ce, p, 0, base + e);
             }
-            x.y *= a.in_sign;
             g[it] = x;
+        }
+        if (a.in_sign < 0.0f) {
+#pragma unroll
+            for (int it = 0; it < R0; ++it) g[it].y = -g[it].y;
         }
         lds_exchange<SPLIT, R0, R0>(
             lds, g, v,
@@ -704,13 +819,15 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
     } else {
         // COLUMN (under the row mapping) or ROW (under the column mapping)
         const Inst ics = decompose<L>(a, inst0 + cs);
-        TwiddleWalk tw;
-        if constexpr (TWID) tw.init(a, ics.l, ps, T);
+        const Walk w = in_walk(a, ics, ps);
 #pragma unroll
-        for (int it = 0; it < R0; ++it) {
-            float2 x = load_elem<L, V, IO>(a, io, ics, ps + it * T);
-            if constexpr (TWID) x = c_mul(x, tw.next());
-            g[it] = x;
+        for (int it = 0; it < R0; ++it) g[it] = load_at<V, IO>(a, io, ics, w, it * T);
+        pre_ops<V, R0>(a, ics, w, g, [](int it) { return it * T; });
+        if constexpr (TWID) {
+            TwiddleWalk tw;
+            tw.init(a, ics.l, ps, T);
+#pragma unroll
+            for (int it = 0; it < R0; ++it) g[it] = c_mul(g[it], tw.next());
         }
         lds_exchange<SPLIT, R0, R0>(
             lds, g, v, [&](int it) { return cs * LP + lds_pad(ps + it * T); }, own_pos);
@@ -721,7 +838,7 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
     auto last_out = [&](int i) {
         return stage_valid<L, SL>(t, i / RL) ? cbase + lds_pad(stage_out_pos<L, SL>(t, i)) : -1;
     };
-    run_stages<L, 0, SPLIT, VM>(v, my, t, a.twl);
+    run_stages<L, 0, SPLIT, VM, WAVE>(v, my, t, a.twl);
 
     if constexpr (BLUE) {
         // A = FFT(chirp * x): multiply by the kernel spectrum, conjugate, and
@@ -732,15 +849,21 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
                 const int k = stage_out_pos<L, SL>(t, i);
                 v[i] = c_conj(c_mul(v[i], a.bspec[k]));
             }
-        lds_exchange<SPLIT, QL * RL, R0>(lds, v, v, last_out, own_pos);
-        run_stages<L, 0, SPLIT, VM>(v, my, t, a.twl);
+        lds_exchange<SPLIT, QL * RL, R0, WAVE>(lds, v, v, last_out, own_pos);
+        run_stages<L, 0, SPLIT, VM, WAVE>(v, my, t, a.twl);
     }
 
     // ---- store: output element k = stage_out_pos of the last stage ----
+    const bool adjust = a.out_sign < 0.0f || a.out_scale != 1.0f;
     if (store_mode == RK_MODE_DIRECT) {
+        // last stage: butterfly j = t + q*T < NS, so output k = t + stage_out_pos(0, i)
+        static_assert(P::nb(SL) == P::ns(SL), "the last Stockham stage writes its outputs at j + r * NS");
+        if (ic.valid) {
+            const Walk w = out_walk(a, ic, t);
 #pragma unroll
-        for (int i = 0; i < QL * RL; ++i)
-            if (stage_valid<L, SL>(t, i / RL)) store_elem<L, V, IO>(a, io, ic, stage_out_pos<L, SL>(t, i), v[i]);
+            for (int i = 0; i < QL * RL; ++i)
+                if (stage_valid<L, SL>(t, i / RL)) store_at<V, IO>(a, io, ic, w, stage_out_pos<L, SL>(0, i), v[i], adjust);
+        }
     } else if (store_mode == RK_MODE_FLAT) {
         // no post-ops beyond sign and scale
         lds_exchange<SPLIT, QL * RL, R0>(lds, v, g, last_out, [&](int it) {
@@ -755,8 +878,10 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
             const int ce = e / L, k = e - ce * L;
             if (inst0 + ce < a.n_inst) {
                 float2 y = g[it];
-                y.y *= a.out_sign;
-                y = c_scale(y, a.out_scale);
+                if (adjust) {
+                    y.y *= a.out_sign;
+                    y = c_scale(y, a.out_scale);
+                }
                 io.store(a, inst0 + ce, k, 0, base + e, y);
             }
         }
@@ -764,8 +889,11 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
         lds_exchange<SPLIT, QL * RL, R0>(lds, v, g, last_out,
                                          [&](int it) { return cs * LP + lds_pad(ps + it * T); });
         const Inst ics = decompose<L>(a, inst0 + cs);
+        if (ics.valid) {
+            const Walk w = out_walk(a, ics, ps);
 #pragma unroll
-        for (int it = 0; it < R0; ++it) store_elem<L, V, IO>(a, io, ics, ps + it * T, g[it]);
+            for (int it = 0; it < R0; ++it) store_at<V, IO>(a, io, ics, w, it * T, g[it], adjust);
+        }
     }
 }
 
@@ -774,13 +902,39 @@ template <int L, int C> __host__ __device__ constexpr int pass_threads() { retur
 
 // One work-group of a pass launch: the tile this work-group owns, with the
 // work-group's own LDS.
-template <int L, int C, int V, class IO, int MODES = 0>
+template <int L, int C, int V, class IO, int MODES = 0, bool WAVE = false>
 __device__ __forceinline__ void fft_pass_body(const rk_pass_args& a, const IO& io) {
     constexpr int LP = tile_pitch<L, C>();
     __shared__ float2 lds[tile_split<L, C>() ? (C * LP + 1) / 2 : C * LP];
     const rk_u64 tile = xcd_tile(a) + a.tile_base;
     if (tile >= a.n_tiles) return;
-    fft_pass_tile<L, C, V, IO, MODES>(a, io, lds, tile, (int)threadIdx.x);
+    fft_pass_tile<L, C, V, IO, MODES, WAVE>(a, io, lds, tile, (int)threadIdx.x);
+}
+
+// The pass kernel of a catalogue variant number V: the low seven bits are the
+// compile-time Variant (pre-op, post-op, Bluestein), the higher bits choose the
+// IO policy (RK_VAR_STREAM, RK_VAR_NT_LOAD, RK_VAR_NT_STORE), compiled-in
+// load/store modes (RK_VAR_MODE_*) and wave-local stages (RK_VAR_WAVE).  The
+// ahead-of-time catalogue and the run-time compiler instantiate this same
+// entry.
+template <int L, int C, int V>
+__device__ __forceinline__ void fft_pass_entry(const rk_pass_args& a) {
+    constexpr int VB = V & (RK_VAR_STREAM - 1);
+    constexpr int M = (V & RK_VAR_MODE_DD) ? RK_MODES(RK_MODE_DIRECT, RK_MODE_DIRECT)
+                    : (V & RK_VAR_MODE_DF) ? RK_MODES(RK_MODE_DIRECT, RK_MODE_FLAT)
+                    : (V & RK_VAR_MODE_FF) ? RK_MODES(RK_MODE_FLAT, RK_MODE_FLAT) : 0;
+    constexpr bool NTL = (V & (RK_VAR_STREAM | RK_VAR_NT_LOAD)) != 0;
+    constexpr bool NTS = (V & (RK_VAR_STREAM | RK_VAR_NT_STORE)) != 0;
+    constexpr bool WAVE = (V & RK_VAR_WAVE) != 0;
+    fft_pass_body<L, C, VB, GlobalIO<NTL, NTS>, M, WAVE>(a, GlobalIO<NTL, NTS>{});
+}
+
+// Minimum waves per SIMD of a pass kernel's launch bounds: three for the
+// single-kernel Bluestein tiles with DIRECT/DIRECT compiled in of at most 256
+// threads (at most 168 VGPRs), else one (the tile's own thread count bounds
+// its registers).
+template <int L, int C, int V> __host__ __device__ constexpr int pass_min_waves() {
+    return ((V & RK_VAR_MODE_DD) && (V & 16) && pass_threads<L, C>() <= 256) ? 3 : 1;
 }
 
 }   // namespace rkfft

@@ -440,6 +440,11 @@ int describe_pass(const AxisPass& ap, const Layout& ls, const Layout& ld, int ax
             return fail(RK_ERR_UNSUPPORTED, "internal: pass with both chirp and twiddle pre-ops");
         pp.variant = pre | (post << 2);
     }
+    // the kernels bound positions one by one only on chirp sides (pad_in /
+    // pad_out of fft_device.hpp): every other side covers its whole axis
+    if ((!(ap.pre_ops & RK_PRE_CHIRP) && ap.n_valid_in != ap.Nax) ||
+        (!(ap.post_ops & RK_POST_CHIRP) && ap.n_valid_out != ap.Nax))
+        return fail(RK_ERR_UNSUPPORTED, "internal: a padded pass side without a chirp op (L=%d)", ap.L);
     rk_pass_args& a = pp.args;
     int64_t O, so_in, so_out, I, si_in, si_out;
     fold_batch(ls, ld, axis, O, so_in, so_out, I, si_in, si_out, pp.loops);
@@ -531,7 +536,7 @@ static void set_xcd_group(PassPlan& pp) {
     rk_u32 g = 1;
     if (forced > 0)
         g = (rk_u32)forced;
-    else if (pp.args.cmap && pp.C < 16)
+    else if ((pp.args.cmap || pp.args.load_mode == RK_MODE_COLUMN || pp.args.store_mode == RK_MODE_COLUMN) && pp.C < 16)
         g = (rk_u32)(32 / pp.C);
     if (g > 1) {
         pp.args.xcd_group = g;
@@ -579,56 +584,93 @@ static int fixed_mode_bit(const PassPlan& pp) {
     return 0;
 }
 
+// Variant bits of a pass's kernel beyond its Variant: the IO policy and the
+// wave-local stages.
+static int io_variant_bits(const PassPlan& pp) {
+    int v = 0;
+    if (pp.nt_load && pp.nt_store)
+        v |= VAR_STREAM;
+    else if (pp.nt_load)
+        v |= VAR_NT_LOAD;
+    else if (pp.nt_store)
+        v |= VAR_NT_STORE;
+    if (pp.wave) v |= VAR_WAVE;
+    return v;
+}
+
+static const KernelEntry* catalog_find(int L, int C, int variant) {
+    for (const KernelEntry& e : catalog())
+        if (e.L == L && e.C == C && e.variant == variant) return &e;
+    return nullptr;
+}
+
+// The pass's kernel: the catalogue's, with the pass's load/store modes
+// compiled in when such a twin exists, else from the run-time compiler (the
+// same entry, any variant number).  Without run-time compilation, a pass
+// whose nontemporal twin is missing keeps the plain accesses.
 static int find_kernel(PassPlan& pp, uint32_t flags) {
     pp.fixed_modes = false;
     const int mbit = fixed_mode_bit(pp);
-    if (pp.stream) {
-        // a twin with this pass's load/store modes compiled in, when there is one
-        if (mbit)
-            for (const KernelEntry& e : catalog())
-                if (e.L == pp.L && e.C == pp.C && e.variant == (pp.variant | VAR_STREAM | mbit)) {
-                    pp.aot = e.func;
-                    pp.threads = e.threads;
-                    pp.lds_bytes = e.lds_bytes;
-                    pp.fixed_modes = true;
-                    return RK_OK;
-                }
-        for (const KernelEntry& e : catalog())
-            if (e.L == pp.L && e.C == pp.C && e.variant == (pp.variant | VAR_STREAM)) {
-                pp.aot = e.func;
-                pp.threads = e.threads;
-                pp.lds_bytes = e.lds_bytes;
-                return RK_OK;
-            }
-        pp.stream = false;   // no streaming twin: the plain kernel
-    }
+    auto take = [&](const KernelEntry* e, bool fixed) {
+        pp.aot = e->func;
+        pp.threads = e->threads;
+        pp.lds_bytes = e->lds_bytes;
+        pp.fixed_modes = fixed;
+    };
+    const int want = pp.variant | io_variant_bits(pp);
     if (mbit)
-        for (const KernelEntry& e : catalog())
-            if (e.L == pp.L && e.C == pp.C && e.variant == (pp.variant | mbit)) {
-                pp.aot = e.func;
-                pp.threads = e.threads;
-                pp.lds_bytes = e.lds_bytes;
-                pp.fixed_modes = true;
-                return RK_OK;
-            }
-    for (const KernelEntry& e : catalog())
-        if (e.L == pp.L && e.C == pp.C && e.variant == pp.variant) {
-            pp.aot = e.func;
-            pp.threads = e.threads;
-            pp.lds_bytes = e.lds_bytes;
-            return RK_OK;
+        if (const KernelEntry* e = catalog_find(pp.L, pp.C, want | mbit)) return take(e, true), RK_OK;
+    if (const KernelEntry* e = catalog_find(pp.L, pp.C, want)) return take(e, false), RK_OK;
+    if (flags & RK_FFT_NO_JIT) {
+        if (pp.nt_load || pp.nt_store) {
+            pp.nt_load = pp.nt_store = false;
+            return find_kernel(pp, flags);
         }
-    if (flags & RK_FFT_NO_JIT)
         return fail(RK_ERR_NO_KERNEL, "no precompiled kernel for L=%d C=%d variant=%d and run-time compilation is off",
-                    pp.L, pp.C, pp.variant);
+                    pp.L, pp.C, want);
+    }
+    pp.fixed_modes = mbit != 0;
     if (flags & RK_FFT_DRY_RUN) {   // describe the kernel the run-time compiler would build
         pp.threads = pp.C * threads_per_instance(pp.L);
         pp.lds_bytes = tile_lds_bytes_host(pp.L, pp.C);
         return RK_OK;
     }
-    int rc = jit_pass_kernel(pp.L, pp.C, pp.variant, &pp.jit, &pp.threads, &pp.lds_bytes);
-    if (rc != RK_OK) return rc;
-    return RK_OK;
+    return jit_pass_kernel(pp.L, pp.C, want | mbit, &pp.jit, &pp.threads, &pp.lds_bytes);
+}
+
+// Passes whose instances fill whole waves (T = L / R0 divides 64) run the
+// row mapping with wave-local stage exchanges: a column-like side is then
+// staged through LDS with lanes across instances (COLUMN), a contiguous
+// side read by each instance's own lanes (DIRECT).  RK_WAVE=1 turns it on.
+static bool wave_enabled() {
+    static const bool v = [] {
+        const char* e = getenv("RK_WAVE");
+        return e && *e && atoi(e) != 0;
+    }();
+    return v;
+}
+
+static void choose_wave(PassPlan& pp, uint32_t flags) {
+    if (!wave_enabled()) return;
+    const int T = threads_per_instance(pp.L);
+    if (T > 64 || 64 % T != 0) return;
+    if (flags & RK_FFT_NO_JIT) {
+        bool have = false;
+        for (const KernelEntry& e : catalog())
+            have = have || (e.L == pp.L && e.C == pp.C && (e.variant & ~(VAR_STREAM | VAR_NT_LOAD | VAR_NT_STORE)) ==
+                                                             (pp.variant | VAR_WAVE));
+        if (!have) return;
+    }
+    rk_pass_args& a = pp.args;
+    if (a.cmap) {
+        auto to_rows = [](rk_u32 m) -> rk_u32 {
+            return m == RK_MODE_DIRECT ? RK_MODE_COLUMN : m == RK_MODE_ROW ? RK_MODE_DIRECT : m;
+        };
+        a.load_mode = to_rows(a.load_mode);
+        a.store_mode = to_rows(a.store_mode);
+        a.cmap = 0;
+    }
+    pp.wave = true;
 }
 
 int build_tables(TableBuilder& tb, PassPlan& pp, const AxisPass& ap, int64_t nreal) {
@@ -819,12 +861,7 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
     size_t ws_elems = 0;
     bool two_ws = false;
     PassHook hook = [&](PassPlan& pp, const AxisPass& ap, int64_t n) -> int {
-        pp.stream = stream_pass(pp);
-        int rc = find_kernel(pp, d->flags);
-        if (rc != RK_OK) return rc;
-        set_xcd_group(pp);
-        const uint64_t max_grid = (uint64_t(1) << 31) / (uint64_t)pp.threads;
-        pp.grid = std::min<uint64_t>(pp.args.n_tiles, max_grid);
+        choose_wave(pp, d->flags);
         return build_tables(tb, pp, ap, n);
     };
     int rc = route_passes(*d, work, lin, lout, default_limits(), plan_tile_width, hook, plan->passes, ws_elems, two_ws);
@@ -838,14 +875,18 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
         const int64_t O = plan->passes.front().outer_count;
         for (auto& p : plan->passes)
             ok = ok && p.loops.empty() && p.outer_count == O && O > 1 && p.inst_per_outer % (uint64_t)p.C == 0;
+        // Chunks of 128 MiB of intermediate for chains whose passes would
+        // otherwise stream the whole array through HBM (more than the 256 MB
+        // Infinity Cache): measured on 64 x 2^20 (three passes) 0.540 against
+        // 0.576 ms, with only the first pass's loads and the last pass's
+        // stores nontemporal; 64 and 192 MiB chunks 0.596 / 0.591 ms.
+        // RK_CHUNK_BYTES overrides (0 turns chunking off).
         static const int64_t chunk_bytes = [] {
             const char* e = getenv("RK_CHUNK_BYTES");
-            // off by default: on MI355X chunks of 32-128 MiB measured no faster
-            // than whole-batch passes (the launches per chunk cost more than
-            // the Infinity Cache saves)
-            return e && *e ? atoll(e) : (int64_t)0;
+            return e && *e ? atoll(e) : (int64_t)128 << 20;
         }();
-        if (ok && chunk_bytes > 0) {
+        const double pass_bytes = 16.0 * (double)plan->passes.front().args.n_inst * plan->passes.front().L;
+        if (ok && chunk_bytes > 0 && stream_min_bytes() > 0 && pass_bytes > stream_min_bytes()) {
             const int64_t per_outer = (int64_t)ws_elems / O;
             const int64_t chunk = std::max<int64_t>(1, chunk_bytes / (per_outer * (int64_t)sizeof(float2)));
             if (chunk < O) {
@@ -854,6 +895,24 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
                 ws_elems = (size_t)(chunk * per_outer);
             }
         }
+    }
+    // the IO policy of every pass, then its kernel.  A chunked chain streams
+    // only what comes from and goes to the caller's arrays (its first pass's
+    // loads, its last pass's stores): the intermediate stays in the Infinity
+    // Cache between the passes of a chunk.
+    for (PassPlan& pp : plan->passes) {
+        const bool plain = pp.variant == VAR_PLAIN || pp.variant == VAR_TWIDDLE;
+        if (plan->chunk_outer) {
+            pp.nt_load = plain && pp.src == BUF_IN && stream_min_bytes() > 0;
+            pp.nt_store = plain && pp.dst == BUF_OUT && stream_min_bytes() > 0;
+        } else {
+            pp.nt_load = pp.nt_store = stream_pass(pp);
+        }
+        rc = find_kernel(pp, d->flags);
+        if (rc != RK_OK) return rc;
+        set_xcd_group(pp);
+        const uint64_t max_grid = (uint64_t(1) << 31) / (uint64_t)pp.threads;
+        pp.grid = std::min<uint64_t>(pp.args.n_tiles, max_grid);
     }
     plan->ws_bytes = 0;
     if (ws_elems) {
@@ -1033,6 +1092,13 @@ extern "C" int rk_fft_plan_num_passes(const rk_fft_plan* plan, int* n) {
     return RK_OK;
 }
 
+extern "C" int rk_fft_plan_chunking(const rk_fft_plan* plan, int64_t* chunk_outer, int64_t* n_outer) {
+    if (!plan || !chunk_outer || !n_outer) return fail(RK_ERR_INVALID, "rk_fft_plan_chunking: null argument");
+    *chunk_outer = plan->chunk_outer;
+    *n_outer = plan->n_outer;
+    return RK_OK;
+}
+
 extern "C" int rk_fft_plan_pass_info(const rk_fft_plan* plan, int pass, rk_fft_pass_info* info) {
     if (!plan || !info) return fail(RK_ERR_INVALID, "rk_fft_plan_pass_info: null argument");
     if (pass < 0 || pass >= (int)plan->passes.size()) return fail(RK_ERR_INVALID, "pass index %d out of range", pass);
@@ -1057,7 +1123,10 @@ extern "C" int rk_fft_plan_pass_info(const rk_fft_plan* plan, int pass, rk_fft_p
     info->lds_bytes = pp.lds_bytes;
     info->jit = pp.jit ? 1 : 0;
     info->variant = pp.variant;
-    info->stream = pp.stream ? 1 : 0;
+    info->stream = (pp.nt_load && pp.nt_store) ? 1 : 0;
+    info->nt_load = pp.nt_load ? 1 : 0;
+    info->nt_store = pp.nt_store ? 1 : 0;
+    info->wave = pp.wave ? 1 : 0;
     info->in_dims[0] = (int64_t)(pp.args.n_inst / (pp.args.dim_I * pp.args.dim_NS * pp.args.dim_H));
     info->in_dims[1] = (int64_t)pp.args.n_valid_in;
     info->in_dims[2] = (int64_t)pp.args.dim_I;

@@ -51,6 +51,9 @@ enum {
     VAR_MODE_DD = 1 << 8,  // catalogue entries: DIRECT/DIRECT load/store modes compiled in (RK_VAR_MODE_DD)
     VAR_MODE_DF = 1 << 9,  // catalogue entries: DIRECT/FLAT load/store modes compiled in (RK_VAR_MODE_DF)
     VAR_MODE_FF = 1 << 10, // catalogue entries: FLAT/FLAT load/store modes compiled in (RK_VAR_MODE_FF)
+    VAR_NT_LOAD = 1 << 11, // catalogue entries: nontemporal loads only (RK_VAR_NT_LOAD)
+    VAR_NT_STORE = 1 << 12, // catalogue entries: nontemporal stores only (RK_VAR_NT_STORE)
+    VAR_WAVE = 1 << 13,    // catalogue entries: row mapping, wave-local stage exchanges (RK_VAR_WAVE)
 };
 
 // All ahead-of-time kernels compiled into this library.
@@ -118,7 +121,9 @@ struct PassPlan {
     hipFunction_t jit = nullptr;    // run-time compiled kernel
     bool user_kernel = false;       // replaced with rk_fft_plan_set_pass_kernel
     bool is_first = false, is_last = false;
-    bool stream = false;            // nontemporal loads/stores (StreamIO kernel)
+    bool nt_load = false;           // nontemporal loads (the kernel's IO policy)
+    bool nt_store = false;          // nontemporal stores
+    bool wave = false;              // row mapping with wave-local stage exchanges (RK_VAR_WAVE)
     bool fixed_modes = false;       // the kernel has this pass's load/store modes compiled in
     uint64_t grid = 0;
     int axis = 0;

@@ -37,13 +37,16 @@ int jit_pass_kernel(int L, int C, int variant, hipFunction_t* fn, int* threads, 
     auto key = std::make_tuple(dev, L, C, variant);
     auto it = cache.find(key);
     if (it == cache.end()) {
+        // the catalogue's entry, for any variant number (IO policy, modes,
+        // wave-local stages included)
         char src[512];
         snprintf(src, sizeof(src),
                  "#include \"fft_device.hpp\"\n"
-                 "extern \"C\" __global__ void __launch_bounds__(%d) rk_jit_pass(rk_pass_args a) {\n"
-                 "    rkfft::fft_pass_body<%d, %d, %d>(a, rkfft::PlainIO{});\n"
+                 "extern \"C\" __global__ void __launch_bounds__(%d, (rkfft::pass_min_waves<%d, %d, %d>()))\n"
+                 "rk_jit_pass(rk_pass_args a) {\n"
+                 "    rkfft::fft_pass_entry<%d, %d, %d>(a);\n"
                  "}\n",
-                 C * T, L, C, variant);
+                 C * T, L, C, variant, L, C, variant);
         std::string code, log;
         int rc = compile_code_object(src, {"fft_args.h", "fft_device.hpp"}, {k_src_fft_args_h, k_src_fft_device_hpp},
                                      "-Xclang -target-feature -Xclang -packed-fp32-ops", &code, &log);

@@ -224,7 +224,8 @@ class _FusedFft:
                 "rk_fused_pass(rk_pass_args a, rk_user_blob lb, rk_user_blob sb) {\n"
                 "    const RkLoadArgs& la = *reinterpret_cast<const RkLoadArgs*>(lb.b);\n"
                 "    const RkStoreArgs& sa = *reinterpret_cast<const RkStoreArgs*>(sb.b);\n"
-                f"    rkfft::fft_pass_body<{info.L}, {info.C}, {info.variant}, RkFusedIO, {modes}>(a, RkFusedIO{{la, sa}});\n"
+                f"    rkfft::fft_pass_body<{info.L}, {info.C}, {info.variant}, RkFusedIO, {modes}, "
+                f"{'true' if info.wave else 'false'}>(a, RkFusedIO{{la, sa}});\n"
                 "}\n"
             )
             return "".join(parts)

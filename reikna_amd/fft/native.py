@@ -39,6 +39,9 @@ class PassInfo:
     variant: int
     stream: bool = False   # nontemporal loads/stores (transform larger than the Infinity Cache)
     fixed_modes: bool = False   # the kernel has the pass's load/store modes compiled in
+    nt_load: bool = False       # nontemporal loads
+    nt_store: bool = False      # nontemporal stores
+    wave: bool = False          # whole instances per wave, wave-local stage exchanges
 
 
 def _element_layout(meta: ArrayMetadata) -> tuple[list[int], int]:
@@ -94,6 +97,13 @@ class NativeFftPlan:
         self.workspace_bytes = int(ws.value)
 
     @property
+    def chunk_rows(self) -> int:
+        """Outer-batch indices per chunk of a chunked plan (0: not chunked)."""
+        c, n = ctypes.c_int64(0), ctypes.c_int64(0)
+        _lib.call("rk_fft_plan_chunking", ctypes.c_void_p(self.handle), ctypes.byref(c), ctypes.byref(n))
+        return int(c.value)
+
+    @property
     def passes(self) -> list[PassInfo]:
         n = ctypes.c_int(0)
         _lib.call("rk_fft_plan_num_passes", ctypes.c_void_p(self.handle), ctypes.byref(n))
@@ -110,6 +120,7 @@ class NativeFftPlan:
                     Ns=info.Ns, lds_bytes=info.lds_bytes, jit=bool(info.jit), variant=info.variant,
                     stream=bool(info.stream),
                     fixed_modes=bool(info.fixed_modes),
+                    nt_load=bool(info.nt_load), nt_store=bool(info.nt_store), wave=bool(info.wave),
                 )
             )
         return out

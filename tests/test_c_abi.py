@@ -73,3 +73,4 @@ def test_missing_library_fails_loudly(tmp_path):
     root = __import__("pathlib").Path(__file__).resolve().parent.parent
     out = subprocess.run([sys.executable, "-c", code], cwd=root, env=env, capture_output=True, text=True, timeout=120)
     assert "raised: RkError" in out.stdout and "not built" in out.stdout, out.stdout + out.stderr
+

@@ -161,12 +161,25 @@ def test_pure_parallel_snippet_and_errors(vdev):
         PureParallel.from_trf(trf, other.input)
 
 
+def chunked(shape, axes, dev):
+    fft = FFT(ArrayMetadata(shape, numpy.complex64), axes=axes)
+    fft.compile(dev)
+    return fft.native_plan.chunk_rows, fft.native_plan.passes
+
+
 def test_plan_streaming_accesses(vdev):
-    # passes whose input and output exceed the 256 MB Infinity Cache stream
-    # (nontemporal kernels); the in-place passes of a 128 MiB 3-D transform
-    # and small transforms keep plain accesses
-    assert all(p.stream for p in passes((64, 2**20), (1,), dev=vdev))
-    assert all(p.stream for p in passes((4096, 12288), (1,), dev=vdev))
+    # chains whose data exceed the 256 MB Infinity Cache run in chunks of
+    # 128 MiB of intermediate: the caller's input is loaded and the output
+    # stored with the nontemporal hint, the intermediate stays cached
+    for shape, rows in (((64, 2**20), 16), ((4096, 12288), 1365)):
+        n, ps = chunked(shape, (1,), vdev)
+        assert n == rows, (shape, n)
+        assert ps[0].nt_load and not ps[0].nt_store and ps[0].src == "in"
+        assert ps[-1].nt_store and not ps[-1].nt_load and ps[-1].dst == "out"
+        assert not any(p.nt_load or p.nt_store for p in ps[1:-1])
+    # the in-place passes of a 128 MiB 3-D transform and small transforms
+    # keep plain accesses, unchunked
+    assert chunked((256, 256, 256), (0, 1, 2), vdev)[0] == 0
     assert not any(p.stream for p in passes((256, 256, 256), (0, 1, 2), dev=vdev))
     assert not any(p.stream for p in passes((1024, 1024), (1,), dev=vdev))
     # Bluestein passes have no streaming twins
@@ -179,7 +192,7 @@ def test_plan_fixed_mode_kernels(vdev):
     # power-of-two passes and passes with other modes keep the generic kernel
     for shape in ((4096, 12288), (512, 3 * 2**16), (256, 5 * 2**16)):
         ps = passes(shape, (1,), dev=vdev)
-        assert len(ps) >= 2 and all(p.stream for p in ps)
+        assert len(ps) >= 2 and ps[0].nt_load and ps[-1].nt_store   # chunked chains
         assert [p.fixed_modes for p in ps] == [p.L & (p.L - 1) != 0 for p in ps]
         assert any(p.fixed_modes for p in ps)
         assert (ps[0].load_mode, ps[0].store_mode) == ("direct", "flat")
