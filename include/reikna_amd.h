@@ -173,6 +173,18 @@ int rk_fft_plan_workspace_size(const rk_fft_plan* plan, size_t* bytes);
 int rk_fft_execute(const rk_fft_plan* plan, void* out, const void* in,
                    int inverse, void* workspace, void* stream);
 
+/* Same as rk_fft_execute, with the library's own device scratch array as the
+   workspace: it calls no hipMalloc/hipFree and allocates nothing.  No
+   counterpart in the reference: for callers that cannot allocate (code under
+   HIP graph capture, tight serving loops, harnesses that forbid allocation).
+   Limit: plans whose workspace (rk_fft_plan_workspace_size) exceeds
+   RK_SCRATCH_BYTES are refused with RK_ERR_INVALID.  The array (one per GPU,
+   held until the process ends) is shared by every call on that GPU and keeps
+   the last call's contents; calls whose work overlaps on one GPU race. */
+#define RK_SCRATCH_BYTES (64ull << 20)
+int rk_fft_execute_scratch(const rk_fft_plan* plan, void* out, const void* in,
+                           int inverse, void* stream);
+
 /* Same, for plans whose first/last pass kernels were replaced by fused
    (transformation-carrying) kernels with rk_fft_plan_set_pass_kernel: the
    byte blobs are copied into the user-data slots of the first pass

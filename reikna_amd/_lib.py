@@ -24,6 +24,7 @@ RK_ERR_UNSUPPORTED = -4
 RK_ERR_BUDGET = -5
 
 RK_MAX_DIMS = 8
+RK_SCRATCH_BYTES = 64 << 20   # rk_fft_execute_scratch: the library's device scratch array
 RK_FUSED_BLOB_BYTES = 512
 
 RK_FFT_FORCE_BLUESTEIN = 0x1
@@ -245,6 +246,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "rk_fft_plan_create": (_I, [ctypes.POINTER(FftDesc), _PP]),
     "rk_fft_plan_workspace_size": (_I, [_P, ctypes.POINTER(_SZ)]),
     "rk_fft_execute": (_I, [_P, _P, _P, _I, _P, _P]),
+    "rk_fft_execute_scratch": (_I, [_P, _P, _P, _I, _P]),
     "rk_fft_execute_fused": (_I, [_P, _P, _P, _I, _P, _P, _P, _SZ, _P, _SZ]),
     "rk_fft_plan_destroy": (_I, [_P]),
     "rk_fft_plan_num_passes": (_I, [_P, ctypes.POINTER(_I)]),

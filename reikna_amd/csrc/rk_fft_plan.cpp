@@ -22,6 +22,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 
 #include "fft_device.hpp"
 #include "rk_internal.hpp"
@@ -1076,6 +1077,56 @@ extern "C" int rk_fft_execute_fused(const rk_fft_plan* plan, void* out, const vo
         return fail(RK_ERR_INVALID, "fused argument blob larger than %d bytes", RK_FUSED_BLOB_BYTES);
     return rk::plan_execute(plan, out, in, inverse, workspace, (hipStream_t)stream, load_blob, load_bytes, store_blob,
                             store_bytes);
+}
+
+// ---------------------------------------------------------------------------
+// Execution that allocates nothing: the library's own device scratch array as
+// the workspace (no counterpart in the reference).  The array is a file-scope
+// device variable of RK_SCRATCH_BYTES, so every GPU the library runs on holds
+// one copy, at its own address, until the process ends.  Host code never
+// passes the array's name to a kernel (on the host it is only a stand-in for
+// the device symbol): the device address comes from hipGetSymbolAddress, once
+// per device with that device current, and is kept per device.  The passes
+// write every workspace element before they read it, so nothing needs
+// clearing between calls; calls whose work overlaps on one GPU (other
+// streams, or the same plan on two streams) share the array and race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char rk_scratch[RK_SCRATCH_BYTES];
+
+static int scratch_address(int device, void** out) {
+    static std::mutex mu;
+    static std::map<int, void*> addr;   // device -> device address of rk_scratch
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(device);
+    if (it == addr.end()) {
+        int cur = 0;
+        RK_HIP(hipGetDevice(&cur));
+        if (cur != device) RK_HIP(hipSetDevice(device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(rk_scratch));
+        if (cur != device) RK_HIP(hipSetDevice(cur));
+        if (e != hipSuccess || !p)
+            return fail(RK_ERR_HIP, "rk_fft_execute_scratch: address of the scratch array on device %d: %s", device,
+                        hipGetErrorString(e));
+        it = addr.emplace(device, p).first;
+    }
+    *out = it->second;
+    return RK_OK;
+}
+
+extern "C" int rk_fft_execute_scratch(const rk_fft_plan* plan, void* out, const void* in, int inverse, void* stream) {
+    if (!plan) return fail(RK_ERR_INVALID, "rk_fft_execute_scratch: null plan");
+    if (plan->ws_bytes > (size_t)RK_SCRATCH_BYTES)
+        return fail(RK_ERR_INVALID,
+                    "rk_fft_execute_scratch: the plan needs %zu bytes of workspace, more than the %llu bytes of the "
+                    "library's scratch array (use rk_fft_execute with a workspace)",
+                    plan->ws_bytes, (unsigned long long)RK_SCRATCH_BYTES);
+    void* ws = nullptr;
+    if (plan->ws_bytes) {
+        int rc = scratch_address(plan->device, &ws);
+        if (rc != RK_OK) return rc;
+    }
+    return rk::plan_execute(plan, out, in, inverse, ws, (hipStream_t)stream, nullptr, 0, nullptr, 0);
 }
 
 extern "C" int rk_fft_plan_destroy(rk_fft_plan* plan) {

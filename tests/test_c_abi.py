@@ -74,3 +74,12 @@ def test_missing_library_fails_loudly(tmp_path):
     out = subprocess.run([sys.executable, "-c", code], cwd=root, env=env, capture_output=True, text=True, timeout=120)
     assert "raised: RkError" in out.stdout and "not built" in out.stdout, out.stdout + out.stderr
 
+
+def test_scratch_limit_matches_header():
+    """The Python constant of the no-allocation entry point's limit is the
+    header's RK_SCRATCH_BYTES."""
+    from reikna_amd import _lib
+
+    text = (ROOT / "include" / "reikna_amd.h").read_text()
+    m = re.search(r"#define RK_SCRATCH_BYTES \((\d+)ull << (\d+)\)", text)
+    assert m and int(m.group(1)) << int(m.group(2)) == _lib.RK_SCRATCH_BYTES
