@@ -279,7 +279,7 @@ def main(argv=None) -> int:
             "distribution": f"{dist}, seed {seed}",
             "directions": "forward + inverse" if fwd_inv else "forward",
             "passes": [{"L": p.L, "C": p.C, "load": p.load_mode, "store": p.store_mode, "nt_load": p.nt_load,
-                        "nt_store": p.nt_store, "wave": p.wave, "jit": p.jit} for p in passes],
+                        "nt_store": p.nt_store, "wave": p.wave, "pipe": p.pipe, "jit": p.jit} for p in passes],
             "chunk_rows": native.chunk_rows,
         },
         "roofline": {

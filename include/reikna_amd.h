@@ -225,6 +225,8 @@ typedef struct {
     int nt_store;         /* 1 if the kernel stores with the nontemporal hint */
     int wave;             /* 1 if each wave holds whole instances and the exchanges
                              between stages stay inside it (row mapping) */
+    int pipe;             /* 1 if the kernel is persistent and loads its next tile
+                             while it computes the current one */
 } rk_fft_pass_info;
 
 int rk_fft_plan_num_passes(const rk_fft_plan* plan, int* n);

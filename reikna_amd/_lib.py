@@ -158,6 +158,7 @@ class FftPassInfo(ctypes.Structure):
         ("nt_load", ctypes.c_int),
         ("nt_store", ctypes.c_int),
         ("wave", ctypes.c_int),
+        ("pipe", ctypes.c_int),
     ]
 
 

@@ -175,7 +175,20 @@ def catalog_entries() -> list[tuple[int, int, int]]:
     # measured 1-2 % slower with them.
     # The same for the plain (non-streaming) column passes of such lengths:
     # strided axes of multi-axis transforms and chains below the cache size.
+    # Power-of-two passes of chunked chains (the 64 x 2^20 chain) got the
+    # same twins once their element addressing was per-thread walks: the
+    # 128-point middle pass of 64 x 2^20 takes 41.7 us per 128 MiB chunk with
+    # DIRECT/DIRECT compiled in against 44.8 us generic (its pattern copy
+    # 38.8 us).
     nt = VAR_STREAM | VAR_NT_LOAD | VAR_NT_STORE
+    for length, c, v in list(ents):
+        pow2 = length & (length - 1) == 0
+        if pow2 and 16 <= length <= FACTOR_MAX and c == tile_c_col(length) and v & ~nt in (VAR_PLAIN, VAR_TWIDDLE):
+            if v & ~nt == VAR_PLAIN:
+                ents.add((length, c, v | VAR_MODE_DF))
+                ents.add((length, c, v | VAR_MODE_DD))
+            else:
+                ents.add((length, c, v | VAR_MODE_DD))
     for length, c, v in list(ents):
         pow2 = length & (length - 1) == 0
         if not pow2 and length <= FACTOR_MAX and c == tile_c_col(length) and v & ~nt in (VAR_PLAIN, VAR_TWIDDLE):

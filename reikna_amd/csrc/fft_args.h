@@ -70,7 +70,11 @@ struct rk_pass_args {
     float in_sign;           // -1 conjugates the input (inverse transform)
     float out_sign;          // -1 conjugates the output
     float out_scale;         // 1/N on the last pass of an inverse transform
-    float _pad1;
+    rk_u32 div_magic;        // 1: the mag_/sh_ divisors below are set (n_inst < 2^31)
+    // x / dim = (mulhi(x, mag) + x) >> sh for every x < 2^31 (round-up
+    // multipliers computed on the host, sh = ceil(log2 dim))
+    rk_u32 mag_I, mag_NS, mag_H;
+    rk_u32 sh_I, sh_NS, sh_H;
 };
 
 // Arguments of fused (transformation-carrying) pass kernels: the leaf

@@ -280,8 +280,8 @@ RK_HD constexpr bool lds_split(int L, int C) { return C * lds_pitch_base(L) * 8 
 // every catalogue tile: 1.2-3x fewer conflict cycles where this applies,
 // averaged over load/store modes and both mappings; the other tiles keep the
 // odd pitch, which the model favours for them.)
-RK_HD constexpr int lds_pitch_for(int L, int C, int T) {
-    const bool split = lds_split(L, C);
+RK_HD constexpr int lds_pitch_for(int L, int C, int T, bool force_split = false) {
+    const bool split = force_split || lds_split(L, C);
     const bool pow2 = C > 0 && (C & (C - 1)) == 0;
     if (!pow2 || C < 2 || T < 16 || C > (split ? 32 : 16)) return lds_pitch_base(L);
     const int m = split ? 64 : 32;
@@ -302,17 +302,20 @@ struct Inst {
     bool valid;
 };
 
+// x / d for x < 2^31 with the host's round-up multiplier m and shift s.
+__device__ __forceinline__ rk_u32 magic_div(rk_u32 x, rk_u32 m, rk_u32 s) { return (__umulhi(x, m) + x) >> s; }
+
 template <int L>
 __device__ __forceinline__ Inst decompose(const rk_pass_args& a, rk_u64 inst) {
     Inst r;
     r.valid = inst < a.n_inst;
     rk_u64 x = r.valid ? inst : 0;
     rk_u64 i = 0, l = 0, h = 0;
-    if (a.n_inst <= 0xffffffffull) {   // 32-bit division is several times cheaper
-        rk_u32 y = (rk_u32)x;
-        if (a.dim_I > 1) { i = y % (rk_u32)a.dim_I; y /= (rk_u32)a.dim_I; }
-        if (a.dim_NS > 1) { l = y % (rk_u32)a.dim_NS; y /= (rk_u32)a.dim_NS; }
-        if (a.dim_H > 1) { h = y % (rk_u32)a.dim_H; y /= (rk_u32)a.dim_H; }
+    if (a.div_magic) {   // a multiply and a shift per division instead of a division sequence
+        rk_u32 y = (rk_u32)x, q;
+        if (a.dim_I > 1) { q = magic_div(y, a.mag_I, a.sh_I); i = y - q * (rk_u32)a.dim_I; y = q; }
+        if (a.dim_NS > 1) { q = magic_div(y, a.mag_NS, a.sh_NS); l = y - q * (rk_u32)a.dim_NS; y = q; }
+        if (a.dim_H > 1) { q = magic_div(y, a.mag_H, a.sh_H); h = y - q * (rk_u32)a.dim_H; y = q; }
         x = y;
     } else {
         if (a.dim_I > 1) { i = x % a.dim_I; x /= a.dim_I; }
@@ -355,6 +358,19 @@ struct TwiddleWalk {
         return r;
     }
 };
+
+// The inter-pass twiddles of one thread's values x[r] at positions p0 + r*dp
+// of instance l.  (An exact FP32 table of W_{NS*L}^j read at index l*p was
+// measured instead: on 64 x 2^20 the 128-point middle pass went from 181 to
+// 226 us, its lanes' table reads scattered over many cache lines per load,
+// while the FP64 walk costs VALU only.)
+template <int NV>
+__device__ __forceinline__ void apply_pass_twiddles(const rk_pass_args& a, rk_u64 l, int p0, int dp, float2* x) {
+    TwiddleWalk tw;
+    tw.init(a, l, (rk_u64)p0, (rk_u64)dp);
+#pragma unroll
+    for (int r = 0; r < NV; ++r) x[r] = c_mul(x[r], tw.next());
+}
 
 // Plain strided element access: the identity IO policy.
 // An IO policy is an object with load(a, o, m, i, addr) and
@@ -454,6 +470,13 @@ struct CoherentIO {
 // Stockham stages stay inside one wave and need no work-group barrier; only
 // the staged global sides (COLUMN, FLAT) exchange across waves.
 #define RK_VAR_WAVE 8192
+// Bit of a catalogue kernel's variant number: the persistent, software-
+// pipelined body (fft_pass_body_pipe; DIRECT loads only).
+#define RK_VAR_PIPE 16384
+// Bit of a catalogue kernel's variant number: the tile's exchanges go through
+// re/im halves even when the whole tile would fit (half the LDS per
+// work-group, twice the resident work-groups).
+#define RK_VAR_HALF 32768
 // MODES template argument of fft_pass_tile: 0 reads the modes from the pass
 // arguments, RK_MODES(load, store) compiles them in.
 #define RK_MODES(load, store) (16 | (int)(load) | ((int)(store) << 2))
@@ -733,10 +756,16 @@ __device__ __forceinline__ rk_u64 xcd_tile(const rk_pass_args& a) {
 }
 
 // LDS bytes of a tile and whether its exchanges are split into halves.
-template <int L, int C> __host__ __device__ constexpr bool tile_split() { return lds_split(L, C); }
-template <int L, int C> __host__ __device__ constexpr int tile_pitch() { return lds_pitch_for(L, C, Plan<L>::T); }
-template <int L, int C> __host__ __device__ constexpr int tile_lds_bytes() {
-    return C * tile_pitch<L, C>() * (tile_split<L, C>() ? 4 : 8);
+// HALF forces the split exchange on a tile that would fit whole (half the LDS,
+// twice the resident work-groups).
+template <int L, int C, bool HALF = false> __host__ __device__ constexpr bool tile_split() {
+    return HALF || lds_split(L, C);
+}
+template <int L, int C, bool HALF = false> __host__ __device__ constexpr int tile_pitch() {
+    return lds_pitch_for(L, C, Plan<L>::T, HALF);
+}
+template <int L, int C, bool HALF = false> __host__ __device__ constexpr int tile_lds_bytes() {
+    return C * tile_pitch<L, C, HALF>() * (tile_split<L, C, HALF>() ? 4 : 8);
 }
 
 // One tile (instances tile*C ... tile*C + C - 1) computed by threads
@@ -745,21 +774,24 @@ template <int L, int C> __host__ __device__ constexpr int tile_lds_bytes() {
 // (a, tile, tid) alone, so a caller with more threads than C*T may give the
 // surplus threads tid = C*T - 1: they repeat that thread's work, writing the
 // same values to the same addresses, and take part in every barrier.
-template <int L, int C, int V, class IO, int MODES = 0, bool WAVE = false>
-__device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& io, float2* lds, rk_u64 tile, int tid) {
+// With PRE, the tile's DIRECT loads were issued earlier (raw values in
+// `pre`, see direct_raw_loads): the tile starts from them instead of loading.
+template <int L, int C, int V, class IO, int MODES = 0, bool WAVE = false, bool PRE = false, bool HALF = false>
+__device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& io, float2* lds, rk_u64 tile, int tid,
+                                              const float2* pre = nullptr) {
     using P = Plan<L>;
     static_assert(!WAVE || (64 % P::T == 0), "wave-local stages need whole instances per wave");
     constexpr int T = P::T;
     constexpr int NT = C * T;
     constexpr int R0 = P::R0;
-    constexpr int LP = tile_pitch<L, C>();
+    constexpr int LP = tile_pitch<L, C, HALF>();
     constexpr int VM = P::vmax();
     constexpr int SL = P::NSTAGE - 1;
     constexpr int RL = P::radix(SL);
     constexpr int QL = P::q(SL);
     constexpr bool BLUE = Variant<V>::BLUE;
     constexpr bool TWID = Variant<V>::PRE == RK_VAR_PRE_TWIDDLE;
-    constexpr bool SPLIT = tile_split<L, C>();
+    constexpr bool SPLIT = tile_split<L, C, HALF>();
     const rk_u64 inst0 = tile * (rk_u64)C;
     const rk_u32 load_mode = MODES ? (rk_u32)(MODES & 3) : a.load_mode;
     const rk_u32 store_mode = MODES ? (rk_u32)((MODES >> 2) & 3) : a.store_mode;
@@ -780,17 +812,17 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
     // ---- stage-0 inputs: element p = t + r*T of instance c ----
     const Inst ic = decompose<L>(a, inst0 + c);
     auto own_pos = [&](int i) { return cbase + lds_pad(stage_in_pos<L, 0>(t, i)); };
-    if (load_mode == RK_MODE_DIRECT) {
+    if (PRE || load_mode == RK_MODE_DIRECT) {
         const Walk w = in_walk(a, ic, t);
+        if constexpr (PRE) {
 #pragma unroll
-        for (int r = 0; r < R0; ++r) v[r] = load_at<V, IO>(a, io, ic, w, r * T);
-        pre_ops<V, R0>(a, ic, w, v, [](int r) { return r * T; });
-        if constexpr (TWID) {
-            TwiddleWalk tw;
-            tw.init(a, ic.l, t, T);
+            for (int r = 0; r < R0; ++r) v[r] = pre[r];
+        } else {
 #pragma unroll
-            for (int r = 0; r < R0; ++r) v[r] = c_mul(v[r], tw.next());
+            for (int r = 0; r < R0; ++r) v[r] = load_at<V, IO>(a, io, ic, w, r * T);
         }
+        pre_ops<V, R0>(a, ic, w, v, [](int r) { return r * T; });
+        if constexpr (TWID) apply_pass_twiddles<R0>(a, ic.l, t, T, v);
     } else if (load_mode == RK_MODE_FLAT) {
         // instance-linear block of C*L elements, no pre-ops
         const rk_i64 base = a.in_off + (rk_i64)inst0 * L;
@@ -823,12 +855,7 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
 #pragma unroll
         for (int it = 0; it < R0; ++it) g[it] = load_at<V, IO>(a, io, ics, w, it * T);
         pre_ops<V, R0>(a, ics, w, g, [](int it) { return it * T; });
-        if constexpr (TWID) {
-            TwiddleWalk tw;
-            tw.init(a, ics.l, ps, T);
-#pragma unroll
-            for (int it = 0; it < R0; ++it) g[it] = c_mul(g[it], tw.next());
-        }
+        if constexpr (TWID) apply_pass_twiddles<R0>(a, ics.l, ps, T, g);
         lds_exchange<SPLIT, R0, R0>(
             lds, g, v, [&](int it) { return cs * LP + lds_pad(ps + it * T); }, own_pos);
     }
@@ -902,13 +929,62 @@ template <int L, int C> __host__ __device__ constexpr int pass_threads() { retur
 
 // One work-group of a pass launch: the tile this work-group owns, with the
 // work-group's own LDS.
-template <int L, int C, int V, class IO, int MODES = 0, bool WAVE = false>
+template <int L, int C, int V, class IO, int MODES = 0, bool WAVE = false, bool HALF = false>
 __device__ __forceinline__ void fft_pass_body(const rk_pass_args& a, const IO& io) {
-    constexpr int LP = tile_pitch<L, C>();
-    __shared__ float2 lds[tile_split<L, C>() ? (C * LP + 1) / 2 : C * LP];
+    constexpr int LP = tile_pitch<L, C, HALF>();
+    __shared__ float2 lds[tile_split<L, C, HALF>() ? (C * LP + 1) / 2 : C * LP];
     const rk_u64 tile = xcd_tile(a) + a.tile_base;
     if (tile >= a.n_tiles) return;
-    fft_pass_tile<L, C, V, IO, MODES, WAVE>(a, io, lds, tile, (int)threadIdx.x);
+    fft_pass_tile<L, C, V, IO, MODES, WAVE, false, HALF>(a, io, lds, tile, (int)threadIdx.x);
+}
+
+// The raw DIRECT loads of one tile (what fft_pass_tile<..., PRE> starts from).
+template <int L, int C, int V, class IO, bool WAVE>
+__device__ __forceinline__ void direct_raw_loads(const rk_pass_args& a, const IO& io, rk_u64 tile, int tid,
+                                                 float2* raw) {
+    using P = Plan<L>;
+    constexpr int T = P::T;
+    const bool colmap = WAVE ? false : a.cmap != 0;
+    const int c = colmap ? (tid % C) : (tid / T);
+    const int t = colmap ? (tid / C) : (tid % T);
+    const Inst ic = decompose<L>(a, tile * (rk_u64)C + c);
+    const Walk w = in_walk(a, ic, t);
+#pragma unroll
+    for (int r = 0; r < P::R0; ++r) raw[r] = load_at<V, IO>(a, io, ic, w, r * T);
+}
+
+// Persistent, software-pipelined pass (DIRECT loads only): work-group b runs
+// the tiles tile_base + b, + G, + 2G, ... below n_tiles (G = gridDim.x) and
+// issues the global loads of its next tile before it computes and stores the
+// current one, so one tile's loads are in flight while another is computed.
+// The host launches about one work-group per resident slot.
+template <int L, int C, int V, class IO, int MODES, bool WAVE, bool HALF>
+__device__ __forceinline__ void fft_pass_body_pipe(const rk_pass_args& a, const IO& io) {
+    constexpr int LP = tile_pitch<L, C, HALF>();
+    constexpr int R0 = Plan<L>::R0;
+    __shared__ float2 lds[tile_split<L, C, HALF>() ? (C * LP + 1) / 2 : C * LP];
+    const int tid0 = (int)threadIdx.x;
+    const rk_u64 G = gridDim.x;
+    rk_u64 tile = a.tile_base + blockIdx.x;
+    if (tile >= a.n_tiles) return;
+    float2 nxt[R0];
+    direct_raw_loads<L, C, V, IO, WAVE>(a, io, tile, tid0, nxt);
+    for (;;) {
+        // everything derived from the thread index (stage twiddle indices and
+        // loads, LDS positions) is recomputed per tile instead of being
+        // hoisted out of the loop and held in registers across tiles: the
+        // empty asm makes the index opaque to the optimiser each iteration
+        int tid = tid0;
+        asm volatile("" : "+v"(tid));
+        float2 cur[R0];
+#pragma unroll
+        for (int r = 0; r < R0; ++r) cur[r] = nxt[r];
+        const rk_u64 next = tile + G;
+        if (next < a.n_tiles) direct_raw_loads<L, C, V, IO, WAVE>(a, io, next, tid, nxt);
+        fft_pass_tile<L, C, V, IO, MODES, WAVE, true, HALF>(a, io, lds, tile, tid, cur);
+        if (next >= a.n_tiles) break;
+        tile = next;
+    }
 }
 
 // The pass kernel of a catalogue variant number V: the low seven bits are the
@@ -926,14 +1002,26 @@ __device__ __forceinline__ void fft_pass_entry(const rk_pass_args& a) {
     constexpr bool NTL = (V & (RK_VAR_STREAM | RK_VAR_NT_LOAD)) != 0;
     constexpr bool NTS = (V & (RK_VAR_STREAM | RK_VAR_NT_STORE)) != 0;
     constexpr bool WAVE = (V & RK_VAR_WAVE) != 0;
-    fft_pass_body<L, C, VB, GlobalIO<NTL, NTS>, M, WAVE>(a, GlobalIO<NTL, NTS>{});
+    constexpr bool HALF = (V & RK_VAR_HALF) != 0;
+    if constexpr ((V & RK_VAR_PIPE) != 0)
+        fft_pass_body_pipe<L, C, VB, GlobalIO<NTL, NTS>, M, WAVE, HALF>(a, GlobalIO<NTL, NTS>{});
+    else
+        fft_pass_body<L, C, VB, GlobalIO<NTL, NTS>, M, WAVE, HALF>(a, GlobalIO<NTL, NTS>{});
 }
 
 // Minimum waves per SIMD of a pass kernel's launch bounds: three for the
 // single-kernel Bluestein tiles with DIRECT/DIRECT compiled in of at most 256
 // threads (at most 168 VGPRs), else one (the tile's own thread count bounds
 // its registers).
+// A persistent (RK_VAR_PIPE) kernel asks for the waves per SIMD its LDS
+// allows, so the registers of its prefetched tile do not cut its occupancy.
 template <int L, int C, int V> __host__ __device__ constexpr int pass_min_waves() {
+    if ((V & RK_VAR_PIPE) != 0) {
+        const int waves = (pass_threads<L, C>() + 63) / 64;
+        const int slots = (160 * 1024) / tile_lds_bytes<L, C, (V & RK_VAR_HALF) != 0>();
+        const int w = (slots * waves) / 4;
+        return w < 1 ? 1 : (w > 8 ? 8 : w);
+    }
     return ((V & RK_VAR_MODE_DD) && (V & 16) && pass_threads<L, C>() <= 256) ? 3 : 1;
 }
 

@@ -426,6 +426,25 @@ static void fold_batch(const Layout& src, const Layout& dst, int axis, int64_t& 
     for (auto& g : inner) loops.push_back(g);
 }
 
+// x / d = (mulhi(x, m) + x) >> s for every x < 2^31, with s = ceil(log2 d)
+// and the round-up multiplier m = floor(2^(32+s) / d) - 2^32 + 1 (< 2^32);
+// mulhi(x, m) <= x keeps the sum below 2^32.
+static void div_magic(rk_u64 d, rk_u32& m, rk_u32& s) {
+    rk_u32 sh = 0;
+    while ((rk_u64(1) << sh) < d) ++sh;
+    m = (rk_u32)(((unsigned __int128)1 << (32 + sh)) / d - ((unsigned __int128)1 << 32) + 1);
+    s = sh;
+}
+
+void set_div_magic(rk_pass_args& a) {
+    a.div_magic = 0;
+    if (a.n_inst >= (rk_u64(1) << 31)) return;   // the kernels divide in 64 bits
+    div_magic(a.dim_I ? a.dim_I : 1, a.mag_I, a.sh_I);
+    div_magic(a.dim_NS ? a.dim_NS : 1, a.mag_NS, a.sh_NS);
+    div_magic(a.dim_H ? a.dim_H : 1, a.mag_H, a.sh_H);
+    a.div_magic = 1;
+}
+
 int describe_pass(const AxisPass& ap, const Layout& ls, const Layout& ld, int axis, TileWidthFn tile_width,
                   PassPlan& pp) {
     memset(&pp.args, 0, sizeof(pp.args));
@@ -505,6 +524,7 @@ int describe_pass(const AxisPass& ap, const Layout& ls, const Layout& ld, int ax
     a.store_mode = side_mode(col_out, flat_out, pos_out);
     pp.C = tile_width(ap.L, a.cmap != 0);
     a.n_tiles = (a.n_inst + (rk_u64)pp.C - 1) / (rk_u64)pp.C;
+    set_div_magic(a);
     pp.inst_per_outer = (uint64_t)(ap.H * ap.NS * I);
     pp.outer_count = O;
     return RK_OK;
@@ -596,6 +616,8 @@ static int io_variant_bits(const PassPlan& pp) {
     else if (pp.nt_store)
         v |= VAR_NT_STORE;
     if (pp.wave) v |= VAR_WAVE;
+    if (pp.pipe) v |= VAR_PIPE;
+    if (pp.half) v |= VAR_HALF;
     return v;
 }
 
@@ -646,6 +668,41 @@ static int find_kernel(PassPlan& pp, uint32_t flags) {
 static bool wave_enabled() {
     static const bool v = [] {
         const char* e = getenv("RK_WAVE");
+        return e && *e && atoi(e) != 0;
+    }();
+    return v;
+}
+
+// Passes whose loads are DIRECT can run the persistent, software-pipelined
+// kernel (each work-group loads its next tile while it computes the current
+// one).  RK_PIPE=1 turns it on.
+static bool pipe_enabled() {
+    static const bool v = [] {
+        const char* e = getenv("RK_PIPE");
+        return e && *e && atoi(e) != 0;
+    }();
+    return v;
+}
+
+// Work-groups of a persistent pass: one per resident slot of every CU, the
+// slots bounded by the tile's LDS and by 32 waves per CU.
+static uint64_t pipe_grid(const PassPlan& pp) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) {
+        (void)hipGetLastError();
+        cus = 256;
+    }
+    const int waves = std::max(1, (pp.threads + 63) / 64);
+    const int by_lds = pp.lds_bytes > 0 ? (160 * 1024) / pp.lds_bytes : 32;
+    const int slots = std::max(1, std::min(by_lds, 32 / waves));
+    return (uint64_t)cus * (uint64_t)slots;
+}
+
+// Tiles exchanged through re/im halves even when they fit whole: RK_HALF=1.
+static bool half_enabled() {
+    static const bool v = [] {
+        const char* e = getenv("RK_HALF");
         return e && *e && atoi(e) != 0;
     }();
     return v;
@@ -876,16 +933,21 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
         const int64_t O = plan->passes.front().outer_count;
         for (auto& p : plan->passes)
             ok = ok && p.loops.empty() && p.outer_count == O && O > 1 && p.inst_per_outer % (uint64_t)p.C == 0;
-        // Chunks of 128 MiB of intermediate for chains whose passes would
-        // otherwise stream the whole array through HBM (more than the 256 MB
-        // Infinity Cache): measured on 64 x 2^20 (three passes) 0.540 against
-        // 0.576 ms, with only the first pass's loads and the last pass's
-        // stores nontemporal; 64 and 192 MiB chunks 0.596 / 0.591 ms.
-        // RK_CHUNK_BYTES overrides (0 turns chunking off).
-        static const int64_t chunk_bytes = [] {
+        // Chains whose passes would otherwise stream the whole array through
+        // HBM (more than the 256 MB Infinity Cache) run in chunks whose
+        // intermediates fill that cache: 256 MiB for a two-pass chain (one
+        // workspace), 128 MiB for longer ones (two workspaces in use), with
+        // only the first pass's loads and the last pass's stores nontemporal.
+        // Measured: 64 x 2^20 (three passes) 0.521 ms in 128 MiB chunks
+        // against 0.576 unchunked, 0.544 / 0.615 in 96 / 160 MiB chunks;
+        // 4096 x 12288 (two passes) 0.284 ms in 256 MiB chunks against 0.300
+        // unchunked, 0.309 / 0.324 in 128 / 64 MiB chunks.  RK_CHUNK_BYTES
+        // overrides (0 turns chunking off).
+        static const int64_t chunk_env = [] {
             const char* e = getenv("RK_CHUNK_BYTES");
-            return e && *e ? atoll(e) : (int64_t)128 << 20;
+            return e && *e ? atoll(e) : (int64_t)-1;
         }();
+        const int64_t chunk_bytes = chunk_env >= 0 ? chunk_env : (two_ws ? (int64_t)128 << 20 : (int64_t)256 << 20);
         const double pass_bytes = 16.0 * (double)plan->passes.front().args.n_inst * plan->passes.front().L;
         if (ok && chunk_bytes > 0 && stream_min_bytes() > 0 && pass_bytes > stream_min_bytes()) {
             const int64_t per_outer = (int64_t)ws_elems / O;
@@ -909,11 +971,14 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
         } else {
             pp.nt_load = pp.nt_store = stream_pass(pp);
         }
+        pp.pipe = pipe_enabled() && pp.args.load_mode == RK_MODE_DIRECT && pp.loops.empty();
+        pp.half = half_enabled() && (int64_t)pp.C * lds_pitch_host(pp.L) * 8 <= 80 * 1024;
         rc = find_kernel(pp, d->flags);
         if (rc != RK_OK) return rc;
         set_xcd_group(pp);
         const uint64_t max_grid = (uint64_t(1) << 31) / (uint64_t)pp.threads;
-        pp.grid = std::min<uint64_t>(pp.args.n_tiles, max_grid);
+        pp.grid = std::min<uint64_t>(pp.args.n_tiles, pp.pipe ? pipe_grid(pp) : max_grid);
+        if (pp.pipe) pp.args.xcd_group = 1;   // the persistent body walks tiles in launch order
     }
     plan->ws_bytes = 0;
     if (ws_elems) {
@@ -955,8 +1020,9 @@ static int launch_pass(const PassPlan& pp, rk_pass_args& a, hipStream_t stream, 
             a.out_off += idx * pp.loops[k].out_str;
         }
         // grids beyond the launch limit are split; the kernel stops at tile t1
+        // (a persistent kernel covers [t0, t1) in one launch)
         a.n_tiles = t1;
-        for (rk_u64 base = t0; base < t1; base += pp.grid) {
+        for (rk_u64 base = t0; base < t1; base += pp.pipe ? (t1 - t0) : pp.grid) {
             a.tile_base = base;
             const unsigned grid = (unsigned)std::min<rk_u64>(pp.grid, t1 - base);
             // fused kernels take (pass args, load blob, store blob)
@@ -1178,6 +1244,7 @@ extern "C" int rk_fft_plan_pass_info(const rk_fft_plan* plan, int pass, rk_fft_p
     info->nt_load = pp.nt_load ? 1 : 0;
     info->nt_store = pp.nt_store ? 1 : 0;
     info->wave = pp.wave ? 1 : 0;
+    info->pipe = pp.pipe ? 1 : 0;
     info->in_dims[0] = (int64_t)(pp.args.n_inst / (pp.args.dim_I * pp.args.dim_NS * pp.args.dim_H));
     info->in_dims[1] = (int64_t)pp.args.n_valid_in;
     info->in_dims[2] = (int64_t)pp.args.dim_I;
@@ -1191,6 +1258,11 @@ extern "C" int rk_fft_plan_set_pass_kernel(rk_fft_plan* plan, int pass, void* fu
     plan->passes[pass].aot = nullptr;
     plan->passes[pass].user_kernel = true;
     plan->passes[pass].fixed_modes = false;
+    if (plan->passes[pass].pipe) {   // the replacement is an ordinary (one tile per work-group) kernel
+        rk::PassPlan& pp = plan->passes[pass];
+        pp.pipe = false;
+        pp.grid = std::min<uint64_t>(pp.args.n_tiles, (uint64_t(1) << 31) / (uint64_t)pp.threads);
+    }
     return RK_OK;
 }
 

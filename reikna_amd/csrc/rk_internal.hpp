@@ -54,6 +54,8 @@ enum {
     VAR_NT_LOAD = 1 << 11, // catalogue entries: nontemporal loads only (RK_VAR_NT_LOAD)
     VAR_NT_STORE = 1 << 12, // catalogue entries: nontemporal stores only (RK_VAR_NT_STORE)
     VAR_WAVE = 1 << 13,    // catalogue entries: row mapping, wave-local stage exchanges (RK_VAR_WAVE)
+    VAR_PIPE = 1 << 14,    // catalogue entries: persistent software-pipelined body (RK_VAR_PIPE)
+    VAR_HALF = 1 << 15,    // catalogue entries: exchanges through re/im halves (RK_VAR_HALF)
 };
 
 // All ahead-of-time kernels compiled into this library.
@@ -124,6 +126,8 @@ struct PassPlan {
     bool nt_load = false;           // nontemporal loads (the kernel's IO policy)
     bool nt_store = false;          // nontemporal stores
     bool wave = false;              // row mapping with wave-local stage exchanges (RK_VAR_WAVE)
+    bool pipe = false;              // persistent software-pipelined kernel: one launch per tile range
+    bool half = false;              // exchanges through re/im halves (half the LDS)
     bool fixed_modes = false;       // the kernel has this pass's load/store modes compiled in
     uint64_t grid = 0;
     int axis = 0;
@@ -166,6 +170,9 @@ int route_passes(const rk_fft_desc& d, const std::vector<int>& work, const Layou
 // Stage twiddles, inter-pass twiddles, chirp and Bluestein spectrum of a pass
 // (nreal: the transform length before Bluestein padding).
 int build_tables(TableBuilder& tb, PassPlan& pp, const AxisPass& ap, int64_t nreal);
+// Host-side multipliers of the kernels' instance decomposition (rk_pass_args
+// mag_* / sh_*), set whenever a pass's dims change.
+void set_div_magic(rk_pass_args& a);
 
 // Tile cases of the batch kernel (rk_batch_kernel.hpp).
 struct BatchCase {

@@ -55,7 +55,7 @@ int jit_pass_kernel(int L, int C, int variant, hipFunction_t* fn, int* threads, 
         RK_HIP(hipModuleLoadData(&k.module, code.data()));
         RK_HIP(hipModuleGetFunction(&k.fn, k.module, "rk_jit_pass"));
         k.threads = C * T;
-        k.lds = tile_lds_bytes_host(L, C);
+        k.lds = (variant & VAR_HALF) ? tile_lds_bytes_host(L, C) / 2 : tile_lds_bytes_host(L, C);
         it = cache.emplace(key, k).first;
     }
     *fn = it->second.fn;

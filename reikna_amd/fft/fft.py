@@ -208,6 +208,7 @@ class _FusedFft:
             parts.append(
                 "struct RkFusedIO {\n"
                 "    static constexpr bool kBranchFree = false;\n"
+
                 "    const RkLoadArgs& la;\n"
                 "    const RkStoreArgs& sa;\n"
                 "    __device__ __forceinline__ float2 load(const rk_pass_args& a, rk_u64 o, rk_u64 m, rk_u64 i,"

@@ -42,6 +42,7 @@ class PassInfo:
     nt_load: bool = False       # nontemporal loads
     nt_store: bool = False      # nontemporal stores
     wave: bool = False          # whole instances per wave, wave-local stage exchanges
+    pipe: bool = False          # persistent kernel that loads its next tile while computing one
 
 
 def _element_layout(meta: ArrayMetadata) -> tuple[list[int], int]:
@@ -121,6 +122,7 @@ class NativeFftPlan:
                     stream=bool(info.stream),
                     fixed_modes=bool(info.fixed_modes),
                     nt_load=bool(info.nt_load), nt_store=bool(info.nt_store), wave=bool(info.wave),
+                    pipe=bool(info.pipe),
                 )
             )
         return out

@@ -171,7 +171,7 @@ def test_plan_streaming_accesses(vdev):
     # chains whose data exceed the 256 MB Infinity Cache run in chunks of
     # 128 MiB of intermediate: the caller's input is loaded and the output
     # stored with the nontemporal hint, the intermediate stays cached
-    for shape, rows in (((64, 2**20), 16), ((4096, 12288), 1365)):
+    for shape, rows in (((64, 2**20), 16), ((4096, 12288), 2730)):
         n, ps = chunked(shape, (1,), vdev)
         assert n == rows, (shape, n)
         assert ps[0].nt_load and not ps[0].nt_store and ps[0].src == "in"
@@ -187,17 +187,16 @@ def test_plan_streaming_accesses(vdev):
 
 
 def test_plan_fixed_mode_kernels(vdev):
-    # the column passes of large multi-pass transforms whose lengths hold a
-    # factor 3 or 5 run kernels with their load/store modes compiled in;
-    # power-of-two passes and passes with other modes keep the generic kernel
-    for shape in ((4096, 12288), (512, 3 * 2**16), (256, 5 * 2**16)):
+    # the column passes of large multi-pass (chunked) transforms run kernels
+    # with their load/store modes compiled in, power-of-two lengths included;
+    # single-pass power-of-two rows keep the generic kernel
+    for shape in ((4096, 12288), (512, 3 * 2**16), (256, 5 * 2**16), (64, 2**20)):
         ps = passes(shape, (1,), dev=vdev)
         assert len(ps) >= 2 and ps[0].nt_load and ps[-1].nt_store   # chunked chains
-        assert [p.fixed_modes for p in ps] == [p.L & (p.L - 1) != 0 for p in ps]
-        assert any(p.fixed_modes for p in ps)
+        assert all(p.fixed_modes for p in ps), shape
         assert (ps[0].load_mode, ps[0].store_mode) == ("direct", "flat")
         assert all((p.load_mode, p.store_mode) == ("direct", "direct") for p in ps[1:])
-    assert not any(p.fixed_modes for p in passes((64, 2**20), (1,), dev=vdev))
+        assert not any(p.jit for p in ps), shape   # all in the ahead-of-time catalogue
     assert not any(p.fixed_modes for p in passes((1024, 1024), (1,), dev=vdev))
     # single-pass rows of 3/5-smooth lengths with >= 32 threads per instance
     for n in (1000, 2000, 1536, 4000):
@@ -207,7 +206,8 @@ def test_plan_fixed_mode_kernels(vdev):
     ps = passes((64, 96), (1,), dev=vdev)
     assert [(p.load_mode, p.store_mode, p.fixed_modes) for p in ps] == [("flat", "flat", True)]
     assert not passes((64, 256), (1,), dev=vdev)[0].fixed_modes   # power of two: generic kernel
-    assert not any(p.fixed_modes for p in passes((256, 256, 256), (0, 1, 2), dev=vdev))
+    # the strided (column) passes of 256^3 have DIRECT/DIRECT compiled in, its row pass not
+    assert [p.fixed_modes for p in passes((256, 256, 256), (0, 1, 2), dev=vdev)] == [False, True, True]
 
 
 def test_plan_xcd_groups_for_narrow_column_tiles(vdev):
