@@ -800,7 +800,7 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
     // instances (WAVE: the row mapping, compiled in)
     const bool colmap = WAVE ? false : a.cmap != 0;
     const int c = colmap ? (tid % C) : (tid / T);
-    const int t = colmap ? (tid / C) : (tid % T);
+    int t = colmap ? (tid / C) : (tid % T);
     const int cbase = c * LP;
     // the other mapping, for a side staged through LDS
     const int cs = colmap ? (tid / T) : (tid % C);
@@ -877,6 +877,12 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
                 v[i] = c_conj(c_mul(v[i], a.bspec[k]));
             }
         lds_exchange<SPLIT, QL * RL, R0, WAVE>(lds, v, v, last_out, own_pos);
+        // the second transform and the stores recompute what depends on the
+        // thread index: the index goes through an empty asm, so the first
+        // transform's stage twiddles and the output chirp values are not held
+        // in registers across both transforms (the 2048-point tile: 162 -> 124
+        // VGPRs, four waves per SIMD instead of two)
+        asm volatile("" : "+v"(t));
         run_stages<L, 0, SPLIT, VM, WAVE>(v, my, t, a.twl);
     }
 
