@@ -55,15 +55,21 @@ WORKLOADS = {
 COMPUTATIONS = ("transpose", "rfft", "irfft", "fftshift", "spectrogram")
 FWD_INV = ("c1",)   # a step is the forward and the inverse transform
 
-# HBM bytes per step (every launch of one step), from rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE runs of this benchmark; FETCH_SIZE reads half of a
-# copy kernel's known read size on this GPU and WRITE_SIZE all of it, so
-# traffic = 2 * FETCH_SIZE + WRITE_SIZE (KiB per dispatch), summed over the
-# step's dispatches.  Each entry names the profile summary it comes from.
+# Memory traffic per step (every launch of one step; for c4 per launch of the
+# batch kernel), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this
+# benchmark: FETCH_SIZE reads half of a copy kernel's known read size on this
+# GPU and WRITE_SIZE all of it, so traffic = 2 * FETCH_SIZE + WRITE_SIZE (KiB
+# per dispatch), summed over the step's dispatches.  These count the L2's
+# requests to the memory fabric, reads served by the Infinity Cache included.
+# Each entry names the profile summary it comes from.
 MEASURED_TRAFFIC = {
-    "c2b": ((2 * 262251.2 + 524301.3 + 2 * 262287.0 + 524290.6 + 2 * 262311.4 + 524293.0) * 1024,
-            "profiles/fft_pass_c2b_64x2p20.md"),
-    "c4": ((2 * 132907.8 + 204343.6) * 1024, "profiles/fft_batch_c4_512_uneven.md"),
+    "c2b": (3.224e9, "profiles/fft_pass_c2b_64x2p20.md"),
+    "c1": (3.41e7, "profiles/fft_pass_c1_1024x1024.md"),
+    "c2a": (5.372e8, "profiles/fft_pass_c2a_256cubed.md"),
+    "c3-1000": (6.58e7, "profiles/fft_pass_c3_4096x1000.md"),
+    "c3-997": (6.651e7, "profiles/fft_pass_c3_4096x997_bluestein.md"),
+    "c3-12288": (1.612e9, "profiles/fft_pass_c3_4096x12288.md"),
+    "c4": (4.383e8, "profiles/fft_batch_c4_512_uneven.md"),
 }
 
 
