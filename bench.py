@@ -689,6 +689,10 @@ def bench_serve(args) -> int:
         sweep[mode] = rows
     best = {m: max(rows, key=lambda x: x["throughput_rps"]) for m, rows in sweep.items()}
     total_flops = sum(sp.flops for sp in specs)
+    # the mix's HBM bound: one read and one write of every request's elements
+    # at 8 TB/s is the least time any path can take for the stream
+    mix_bytes = 16.0 * sum(sp.elements for sp in specs)
+    bound_rps = n_req / (mix_bytes / (HBM_PEAK_GBS * 1e9))
     # the same servers on a stream of small requests (1-D 2^6..2^12, 2-D
     # 32^2..128^2, batch 1..16), where per-request launches dominate the
     # plain path
@@ -734,6 +738,13 @@ def bench_serve(args) -> int:
         "sweep": sweep,
         "best": best,
         "speedup_vs_plain": round(best["batched"]["throughput_rps"] / best["plain"]["throughput_rps"], 2),
+        "hbm_bound": {
+            "mix_bytes": mix_bytes,
+            "bound_rps": round(bound_rps, 1),
+            "batched_frac_of_bound": round(best["batched"]["throughput_rps"] / bound_rps, 4),
+            "plain_frac_of_bound": round(best["plain"]["throughput_rps"] / bound_rps, 4),
+            "max_speedup_over_plain": round(bound_rps / best["plain"]["throughput_rps"], 2),
+        },
         "small_mix": {
             "workload": "1-D 2^6..2^12 and 2-D 32^2..128^2, batch 1..16 (log-uniform), arrivals seed 11",
             "sweep": small,

@@ -843,6 +843,8 @@ int route_passes(const rk_fft_desc& d, const std::vector<int>& work, const Layou
             pp.dst = dst;
             rc = describe_pass(ap, ls, ld, axis, tile_width, pp);
             if (rc != RK_OK) return rc;
+            pp.src_stride0 = ls.strides[0];
+            pp.dst_stride0 = ld.strides[0];
             rc = hook(pp, ap, n);
             if (rc != RK_OK) return rc;
             passes.push_back(pp);
@@ -927,17 +929,25 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
     plan->passes.front().is_first = true;
     plan->passes.back().is_last = true;
 
-    // chunked execution (see rk_fft_plan::chunk_outer)
-    if (work.size() == 1 && plan->passes.size() >= 2 && ws_elems) {
+    // chunked execution (see rk_fft_plan::chunk_outer): over dim 0 when it is
+    // a batch dimension and every pass walks it as the outermost factor of
+    // its instance index (no host loops), so that the instances of batch
+    // index b are a contiguous range of whole tiles in every pass
+    if (plan->passes.size() >= 2 && d->shape[0] > 1 && std::find(work.begin(), work.end(), 0) == work.end()) {
+        const int64_t B = d->shape[0];
         bool ok = true;
-        const int64_t O = plan->passes.front().outer_count;
-        for (auto& p : plan->passes)
-            ok = ok && p.loops.empty() && p.outer_count == O && O > 1 && p.inst_per_outer % (uint64_t)p.C == 0;
+        int64_t per_row_elems = 1;   // largest src/dst elements per dim-0 index
+        for (auto& p : plan->passes) {
+            const uint64_t per0 = p.args.n_inst / (uint64_t)B;
+            ok = ok && p.loops.empty() && p.args.n_inst % (uint64_t)B == 0 && per0 % (uint64_t)p.C == 0;
+            p.inst_per_outer = per0;   // instances per dim-0 index
+            per_row_elems = std::max(per_row_elems, std::max(p.src_stride0, p.dst_stride0));
+        }
         // Chains whose passes would otherwise stream the whole array through
         // HBM (more than the 256 MB Infinity Cache) run in chunks whose
         // intermediates fill that cache: 256 MiB for a two-pass chain (one
         // workspace), 128 MiB for longer ones (two workspaces in use), with
-        // only the first pass's loads and the last pass's stores nontemporal.
+        // only the plan's first loads and last stores nontemporal.
         // Measured: 64 x 2^20 (three passes) 0.521 ms in 128 MiB chunks
         // against 0.576 unchunked, 0.544 / 0.615 in 96 / 160 MiB chunks;
         // 4096 x 12288 (two passes) 0.284 ms in 256 MiB chunks against 0.300
@@ -950,12 +960,12 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
         const int64_t chunk_bytes = chunk_env >= 0 ? chunk_env : (two_ws ? (int64_t)128 << 20 : (int64_t)256 << 20);
         const double pass_bytes = 16.0 * (double)plan->passes.front().args.n_inst * plan->passes.front().L;
         if (ok && chunk_bytes > 0 && stream_min_bytes() > 0 && pass_bytes > stream_min_bytes()) {
-            const int64_t per_outer = (int64_t)ws_elems / O;
-            const int64_t chunk = std::max<int64_t>(1, chunk_bytes / (per_outer * (int64_t)sizeof(float2)));
-            if (chunk < O) {
+            const int64_t chunk = std::max<int64_t>(1, chunk_bytes / (per_row_elems * (int64_t)sizeof(float2)));
+            if (chunk < B) {
                 plan->chunk_outer = chunk;
-                plan->n_outer = O;
-                ws_elems = (size_t)(chunk * per_outer);
+                plan->n_outer = B;
+                // the workspace holds one chunk: its layouts are contiguous with dim 0 outermost
+                if (ws_elems) ws_elems = (size_t)((int64_t)ws_elems / B * chunk);
             }
         }
     }
@@ -966,8 +976,8 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
     for (PassPlan& pp : plan->passes) {
         const bool plain = pp.variant == VAR_PLAIN || pp.variant == VAR_TWIDDLE;
         if (plan->chunk_outer) {
-            pp.nt_load = plain && pp.src == BUF_IN && stream_min_bytes() > 0;
-            pp.nt_store = plain && pp.dst == BUF_OUT && stream_min_bytes() > 0;
+            pp.nt_load = plain && pp.is_first && pp.src == BUF_IN && stream_min_bytes() > 0;
+            pp.nt_store = plain && pp.is_last && pp.dst == BUF_OUT && stream_min_bytes() > 0;
         } else {
             pp.nt_load = pp.nt_store = stream_pass(pp);
         }
@@ -1101,8 +1111,8 @@ static int plan_execute(const rk_fft_plan* plan, void* out, const void* in, int 
         for (size_t p = 0; p < np; ++p) {
             const PassPlan& pp = plan->passes[p];
             rk_pass_args a = args[p];
-            if (pp.src == BUF_WS0 || pp.src == BUF_WS1) a.in_off -= o0 * a.in_so;
-            if (pp.dst == BUF_WS0 || pp.dst == BUF_WS1) a.out_off -= o0 * a.out_so;
+            if (pp.src == BUF_WS0 || pp.src == BUF_WS1) a.in_off -= o0 * pp.src_stride0;
+            if (pp.dst == BUF_WS0 || pp.dst == BUF_WS1) a.out_off -= o0 * pp.dst_stride0;
             const rk_u64 t0 = (rk_u64)o0 * pp.inst_per_outer / (rk_u64)pp.C;
             const rk_u64 t1 = (rk_u64)o1 * pp.inst_per_outer / (rk_u64)pp.C;
             const size_t ev = 2 * (chunk_idx * np + p);

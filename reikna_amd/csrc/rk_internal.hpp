@@ -136,6 +136,7 @@ struct PassPlan {
     std::vector<HostLoop> loops;
     uint64_t inst_per_outer = 0;   // H * NS * I: instances per outer-batch index
     int64_t outer_count = 1;       // O
+    int64_t src_stride0 = 0, dst_stride0 = 0;   // element stride of dim 0 in the src / dst layouts
 };
 
 // Device tables of a plan, appended once per key and 256-byte aligned.

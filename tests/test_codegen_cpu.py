@@ -242,3 +242,18 @@ def test_array_arguments_checked_against_the_compiled_layout():
         _check_array_argument("a", arr(meta, 4096), meta)
     with pytest.raises(ValueError, match="shape"):
         _check_array_argument("a", arr(ArrayMetadata((256,), numpy.complex64), 8192), meta)
+
+
+def test_plan_chunks_batched_multi_axis(vdev):
+    # a batch of 2-D transforms larger than the Infinity Cache runs in chunks
+    # of the batch dimension: the row pass writes the output, the column pass
+    # re-reads it from the cache, so only the plan's first loads and last
+    # stores are nontemporal
+    n, ps = chunked((64, 2048, 2048), (1, 2), vdev)
+    assert n == 8 and len(ps) == 2
+    assert ps[0].nt_load and not ps[0].nt_store and ps[1].nt_store and not ps[1].nt_load
+    # a multi-pass axis (workspace) inside a batched 2-D transform
+    n, ps = chunked((4, 8192, 2048), (1, 2), vdev)
+    assert n == 2 and len(ps) == 3
+    # the transformed leading axis cannot be chunked
+    assert chunked((2**14, 4096), (0, 1), vdev)[0] == 0

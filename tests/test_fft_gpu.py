@@ -248,6 +248,24 @@ def test_fused_input_on_every_plan_kind(queue, shape, axes, rows, inverse):
     assert err <= tol, f"{shape} over {axes}: rel-L2 {err:.3g} > {tol:.3g}"
 
 
+@pytest.mark.parametrize("shape,rows", [((16, 2048, 2048), (0, 15)), ((4, 8192, 2048), (1, 3))],
+                         ids=["16x2048^2", "4x8192x2048"])
+def test_chunked_batched_2d(queue, shape, rows):
+    """Batches of 2-D transforms larger than the Infinity Cache run in chunks
+    of the batch dimension (the second with a two-pass axis through the
+    workspace); forward and inverse, in and out of place."""
+    import reikna_amd._lib  # noqa: F401
+
+    data = oracle.test_array(shape, seed=61)
+    for inverse, inplace in ((False, False), (True, True)):
+        res, fft = run_fft(queue, data, (1, 2), inverse=inverse, inplace=inplace)
+        assert fft.native_plan.chunk_rows > 0
+        trf = oracle.ifftn if inverse else oracle.fftn
+        for r in rows:
+            err = oracle.rel_l2(res[r], trf(data[r], (0, 1), workers=16))
+            assert err <= oracle.tolerance(shape, (1, 2)), (shape, inverse, r, err)
+
+
 def test_strided_input_and_output(queue):
     """Non-contiguous arrays: reikna honours arbitrary strides (transformation_modules.py:32-69)."""
     base = oracle.test_array((64, 2048), seed=9)
