@@ -477,6 +477,10 @@ struct CoherentIO {
 // re/im halves even when the whole tile would fit (half the LDS per
 // work-group, twice the resident work-groups).
 #define RK_VAR_HALF 32768
+// Bit of a catalogue kernel's variant number: the launch bounds ask for as
+// many resident work-groups per CU as the tile's LDS allows, so the register
+// budget does not cut the residency to fewer.
+#define RK_VAR_OCC 65536
 // MODES template argument of fft_pass_tile: 0 reads the modes from the pass
 // arguments, RK_MODES(load, store) compiles them in.
 #define RK_MODES(load, store) (16 | (int)(load) | ((int)(store) << 2))
@@ -1019,12 +1023,14 @@ __device__ __forceinline__ void fft_pass_entry(const rk_pass_args& a) {
 // single-kernel Bluestein tiles with DIRECT/DIRECT compiled in of at most 256
 // threads (at most 168 VGPRs), else one (the tile's own thread count bounds
 // its registers).
-// A persistent (RK_VAR_PIPE) kernel asks for the waves per SIMD its LDS
-// allows, so the registers of its prefetched tile do not cut its occupancy.
+// An RK_VAR_OCC kernel asks for the waves per SIMD its LDS allows, so
+// registers do not cut the occupancy; a persistent (RK_VAR_PIPE) kernel
+// without it for one resident work-group per CU (the registers of its
+// prefetched tile then come out of that work-group's share).
 template <int L, int C, int V> __host__ __device__ constexpr int pass_min_waves() {
-    if ((V & RK_VAR_PIPE) != 0) {
+    if ((V & (RK_VAR_PIPE | RK_VAR_OCC)) != 0) {
         const int waves = (pass_threads<L, C>() + 63) / 64;
-        const int slots = (160 * 1024) / tile_lds_bytes<L, C, (V & RK_VAR_HALF) != 0>();
+        const int slots = (V & RK_VAR_OCC) ? (160 * 1024) / tile_lds_bytes<L, C, (V & RK_VAR_HALF) != 0>() : 1;
         const int w = (slots * waves) / 4;
         return w < 1 ? 1 : (w > 8 ? 8 : w);
     }

@@ -618,6 +618,7 @@ static int io_variant_bits(const PassPlan& pp) {
     if (pp.wave) v |= VAR_WAVE;
     if (pp.pipe) v |= VAR_PIPE;
     if (pp.half) v |= VAR_HALF;
+    if (pp.occ) v |= VAR_OCC;
     return v;
 }
 
@@ -685,7 +686,8 @@ static bool pipe_enabled() {
 }
 
 // Work-groups of a persistent pass: one per resident slot of every CU, the
-// slots bounded by the tile's LDS and by 32 waves per CU.
+// slots bounded by the tile's LDS and by 32 waves per CU with registers
+// bounded for that residency (occ), else one per CU.
 static uint64_t pipe_grid(const PassPlan& pp) {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -695,7 +697,7 @@ static uint64_t pipe_grid(const PassPlan& pp) {
     }
     const int waves = std::max(1, (pp.threads + 63) / 64);
     const int by_lds = pp.lds_bytes > 0 ? (160 * 1024) / pp.lds_bytes : 32;
-    const int slots = std::max(1, std::min(by_lds, 32 / waves));
+    const int slots = pp.occ ? std::max(1, std::min(by_lds, 32 / waves)) : 1;
     return (uint64_t)cus * (uint64_t)slots;
 }
 
@@ -704,6 +706,16 @@ static bool half_enabled() {
     static const bool v = [] {
         const char* e = getenv("RK_HALF");
         return e && *e && atoi(e) != 0;
+    }();
+    return v;
+}
+
+// Passes of at least RK_OCC points run kernels whose registers are bounded
+// so that as many work-groups as their LDS allows stay resident (0: off).
+static int occ_min_length() {
+    static const int v = [] {
+        const char* e = getenv("RK_OCC");
+        return e && *e ? atoi(e) : 0;
     }();
     return v;
 }
@@ -983,6 +995,7 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
         }
         pp.pipe = pipe_enabled() && pp.args.load_mode == RK_MODE_DIRECT && pp.loops.empty();
         pp.half = half_enabled() && (int64_t)pp.C * lds_pitch_host(pp.L) * 8 <= 80 * 1024;
+        pp.occ = occ_min_length() > 0 && pp.L >= occ_min_length();
         rc = find_kernel(pp, d->flags);
         if (rc != RK_OK) return rc;
         set_xcd_group(pp);

@@ -56,6 +56,7 @@ enum {
     VAR_WAVE = 1 << 13,    // catalogue entries: row mapping, wave-local stage exchanges (RK_VAR_WAVE)
     VAR_PIPE = 1 << 14,    // catalogue entries: persistent software-pipelined body (RK_VAR_PIPE)
     VAR_HALF = 1 << 15,    // catalogue entries: exchanges through re/im halves (RK_VAR_HALF)
+    VAR_OCC = 1 << 16,     // catalogue entries: registers bounded for the LDS-allowed residency (RK_VAR_OCC)
 };
 
 // All ahead-of-time kernels compiled into this library.
@@ -128,6 +129,7 @@ struct PassPlan {
     bool wave = false;              // row mapping with wave-local stage exchanges (RK_VAR_WAVE)
     bool pipe = false;              // persistent software-pipelined kernel: one launch per tile range
     bool half = false;              // exchanges through re/im halves (half the LDS)
+    bool occ = false;               // registers bounded for the LDS-allowed residency
     bool fixed_modes = false;       // the kernel has this pass's load/store modes compiled in
     uint64_t grid = 0;
     int axis = 0;
