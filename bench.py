@@ -18,6 +18,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import math
 import os
@@ -145,7 +146,8 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="c2b", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--autotune", action="store_true", help="c4: pick batch planner knobs by measurement")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick planner knobs by measurement (c4: batch knobs; single transforms: plan knobs)")
     ap.add_argument("--requests", type=int, default=1000, help="c5: requests per rate point")
     ap.add_argument("--rates", default="1000,2000,5000,10000,20000,50000", help="c5: offered rates (requests/s)")
     ap.add_argument("--small-rates", default="20000,50000,100000,200000,400000,800000",
@@ -179,7 +181,14 @@ def main(argv=None) -> int:
     data = oracle.test_array(shape, seed=seed, dist=dist)
     arr_in = Array.from_host(queue, data)
     arr_out = Array.empty_like(dev, arr_in)
-    fft = FFT(arr_in, axes=axes)
+    tuning, tuned = None, []
+    if args.autotune:
+        from reikna_amd.fft.autotune import autotune_fft
+
+        print("[bench] autotuning the plan ...", file=sys.stderr, flush=True)
+        tuning, tuned = autotune_fft(queue, shape, axes, use_cache=False)
+        print(f"[bench] chosen {tuning}", file=sys.stderr, flush=True)
+    fft = FFT(arr_in, axes=axes, tuning=tuning)
     fftc = fft.compile(dev)
     native = fft.native_plan
     passes = native.passes
@@ -287,6 +296,8 @@ def main(argv=None) -> int:
             "passes": [{"L": p.L, "C": p.C, "load": p.load_mode, "store": p.store_mode, "nt_load": p.nt_load,
                         "nt_store": p.nt_store, "wave": p.wave, "pipe": p.pipe, "jit": p.jit} for p in passes],
             "chunk_rows": native.chunk_rows,
+            **({"tuning": dataclasses.asdict(tuning),
+                "autotune": [[dataclasses.asdict(t), round(ms, 4)] for t, ms in tuned]} if args.autotune else {}),
         },
         "roofline": {
             "bound": "hbm",

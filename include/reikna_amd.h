@@ -159,6 +159,28 @@ typedef struct {
    kweights persistent array (fft.py:216-236, :691-692). */
 int rk_fft_plan_create(const rk_fft_desc* desc, rk_fft_plan** plan);
 
+/* Planner knobs of one plan (rk_fft_plan_create_tuned).  Every field has a
+   "planner's choice" value; a knob the plan cannot honour (a first factor that
+   does not divide an axis, a tile wider than 1024 threads or 159 KB of LDS)
+   is ignored.  Replaces reikna's fixed local-size limits and their halving
+   retry (reikna/fft/fft.py:700-756) with knobs a measured search
+   (reikna_amd.fft.autotune.autotune_fft) chooses. */
+typedef struct {
+    int32_t factor_max;     /* longest pass of a multi-pass chain (0: 256) */
+    int32_t first_factor;   /* length of the first pass of every multi-pass axis (0: the balanced split) */
+    int32_t col_tile;       /* instances per tile of column-mapped passes (0: planner's) */
+    int32_t row_tile;       /* instances per tile of row-mapped passes (0: planner's) */
+    int64_t chunk_bytes;    /* chunk size of chained passes (-1: planner's, 0: no chunks) */
+    int32_t fixed_modes;    /* kernels with load/store modes compiled in (-1: planner's, 0: off, 1: on) */
+    int32_t reserved;       /* 0 */
+} rk_fft_tuning;
+
+/* rk_fft_plan_create with planner knobs (NULL: the planner's choices).  The
+   RK_ROW_TILE / RK_COL_TILE / RK_FACTOR_MAX / RK_CHUNK_BYTES /
+   RK_FIXED_MODES environment overrides apply only where a knob is left at
+   the planner's choice. */
+int rk_fft_plan_create_tuned(const rk_fft_desc* desc, const rk_fft_tuning* tuning, rk_fft_plan** plan);
+
 /* Bytes of scratch device memory rk_fft_execute needs (0 if none).
    Replaces the plan's temp_array allocations (computation.py:404-434,
    fft.py:681-689). */

@@ -132,6 +132,20 @@ class FftDesc(ctypes.Structure):
     ]
 
 
+class FftTuning(ctypes.Structure):
+    """rk_fft_tuning: planner knobs of one plan (0 / -1: the planner's choice)."""
+
+    _fields_ = [
+        ("factor_max", ctypes.c_int32),
+        ("first_factor", ctypes.c_int32),
+        ("col_tile", ctypes.c_int32),
+        ("row_tile", ctypes.c_int32),
+        ("chunk_bytes", ctypes.c_int64),
+        ("fixed_modes", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
 class FftPassInfo(ctypes.Structure):
     _fields_ = [
         ("L", ctypes.c_int),
@@ -245,6 +259,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
         [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, _P, _P, _SZ],
     ),
     "rk_fft_plan_create": (_I, [ctypes.POINTER(FftDesc), _PP]),
+    "rk_fft_plan_create_tuned": (_I, [ctypes.POINTER(FftDesc), ctypes.POINTER(FftTuning), _PP]),
     "rk_fft_plan_workspace_size": (_I, [_P, ctypes.POINTER(_SZ)]),
     "rk_fft_execute": (_I, [_P, _P, _P, _I, _P, _P]),
     "rk_fft_execute_scratch": (_I, [_P, _P, _P, _I, _P]),

@@ -31,6 +31,11 @@ using rk::fail;
 
 namespace rk {
 
+// Knobs of the plan being built on this thread (rk_fft_plan_create_tuned;
+// null: the planner's own choices).  Set for the duration of one plan
+// creation only.
+static thread_local const rk_fft_tuning* t_tuning = nullptr;
+
 // Longest single-pass DFT of a smooth length, and longest single-kernel
 // Bluestein convolution (RK_SINGLE_MAX overrides both, at most 16384).  Above
 // 4096 points a single-pass tile is one instance of 512+ threads with 64+ KB
@@ -104,6 +109,8 @@ ChainLimits default_limits() {
     l.blue_single_max = blue_single_max();
     l.length_ok = nullptr;
     l.strided_single_max = 0;
+    if (t_tuning && t_tuning->factor_max > 0) l.factor_max = t_tuning->factor_max;
+    if (t_tuning && t_tuning->first_factor > 0) l.first_factor = t_tuning->first_factor;
     return l;
 }
 
@@ -142,30 +149,43 @@ int tile_lds_bytes_host(int L, int C) {
 // catalog come from the run-time compiler): RK_ROW_TILE / RK_COL_TILE force
 // the instances per row / column tile when the tile stays within 1024
 // threads and 159 KB of LDS.
-static int forced_tile(const char* name, int L) {
-    const char* e = getenv(name);
-    const int c = e && *e ? atoi(e) : 0;
+// A requested tile width c of length-L instances, or 0 when that tile would
+// exceed 1024 threads or 159 KB of LDS.
+static int tile_if_fits(int c, int L) {
     if (c <= 0) return 0;
     if ((int64_t)c * threads_per_instance(L) > 1024 || (int64_t)c * lds_pitch_host(L) * 8 > 159 * 1024) return 0;
     return c;
 }
 
-int tile_c_row(int L) {
-    static const bool any = getenv("RK_ROW_TILE") != nullptr;
-    if (any)
-        if (int f = forced_tile("RK_ROW_TILE", L)) return f;
+static int forced_tile(const char* name, int L) {
+    const char* e = getenv(name);
+    return tile_if_fits(e && *e ? atoi(e) : 0, L);
+}
+
+static int default_tile_c_row(int L) {
     int T = threads_per_instance(L);
     int C = std::max(1, 256 / T);
     while (C > 1 && (int64_t)C * lds_pitch_host(L) * 8 > 64 * 1024) C /= 2;
     return C;
 }
 
+int tile_c_row(int L) {
+    if (t_tuning && t_tuning->row_tile > 0)
+        if (int f = tile_if_fits(t_tuning->row_tile, L)) return f;
+    static const bool any = getenv("RK_ROW_TILE") != nullptr;
+    if (any)
+        if (int f = forced_tile("RK_ROW_TILE", L)) return f;
+    return default_tile_c_row(L);
+}
+
 int tile_c_col(int L) {
+    if (t_tuning && t_tuning->col_tile > 0)
+        if (int f = tile_if_fits(t_tuning->col_tile, L)) return f;
     static const bool any = getenv("RK_COL_TILE") != nullptr;
     if (any)
         if (int f = forced_tile("RK_COL_TILE", L)) return f;
     int T = threads_per_instance(L);
-    int C = std::max(16, (tile_c_row(L) / 16) * 16);
+    int C = std::max(16, (default_tile_c_row(L) / 16) * 16);
     while (C > 1 && ((int64_t)C * T > 1024 || (int64_t)C * lds_pitch_host(L) * 8 > 159 * 1024)) C /= 2;
     // tuning override: an upper bound on the column tile width (kernels outside
     // the catalog come from the run-time compiler)
@@ -282,6 +302,13 @@ static bool split_factors(int64_t n, const ChainLimits& lim, std::vector<int64_t
         double cap = 1.0;
         for (int k = 0; k < p; ++k) cap *= (double)fmax;
         if (cap < (double)n) continue;
+        // a requested first pass length that divides n (else the balanced split)
+        const int64_t ff = lim.first_factor;
+        if (ff > 1 && ff <= fmax && n % ff == 0 && length_ok(lim, ff)) {
+            out.push_back(ff);
+            if (split_into(n / ff, p - 1, lim, out)) return true;
+            out.clear();
+        }
         if (split_into(n, p, lim, out)) return true;
         out.clear();
     }
@@ -587,6 +614,7 @@ static bool stream_pass(const PassPlan& pp) {
 // Kernels with the pass's load/store modes compiled in (RK_FIXED_MODES=0
 // turns them off).
 static bool use_fixed_modes() {
+    if (t_tuning && t_tuning->fixed_modes >= 0) return t_tuning->fixed_modes != 0;
     static const bool v = [] {
         const char* e = getenv("RK_FIXED_MODES");
         return !(e && *e && atoi(e) == 0);
@@ -969,7 +997,9 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
             const char* e = getenv("RK_CHUNK_BYTES");
             return e && *e ? atoll(e) : (int64_t)-1;
         }();
-        const int64_t chunk_bytes = chunk_env >= 0 ? chunk_env : (two_ws ? (int64_t)128 << 20 : (int64_t)256 << 20);
+        const int64_t chunk_bytes = (t_tuning && t_tuning->chunk_bytes >= 0) ? t_tuning->chunk_bytes
+                                    : chunk_env >= 0 ? chunk_env
+                                    : (two_ws ? (int64_t)128 << 20 : (int64_t)256 << 20);
         const double pass_bytes = 16.0 * (double)plan->passes.front().args.n_inst * plan->passes.front().L;
         if (ok && chunk_bytes > 0 && stream_min_bytes() > 0 && pass_bytes > stream_min_bytes()) {
             const int64_t chunk = std::max<int64_t>(1, chunk_bytes / (per_row_elems * (int64_t)sizeof(float2)));
@@ -1145,6 +1175,21 @@ static int plan_execute(const rk_fft_plan* plan, void* out, const void* in, int 
 // ---------------------------------------------------------------------------
 
 extern "C" int rk_fft_plan_create(const rk_fft_desc* desc, rk_fft_plan** plan) {
+    return rk::plan_create(desc, plan);
+}
+
+extern "C" int rk_fft_plan_create_tuned(const rk_fft_desc* desc, const rk_fft_tuning* tuning, rk_fft_plan** plan) {
+    if (tuning) {
+        if (tuning->factor_max < 0 || tuning->first_factor < 0 || tuning->col_tile < 0 || tuning->row_tile < 0 ||
+            tuning->chunk_bytes < -1 || tuning->fixed_modes < -1 || tuning->fixed_modes > 1 || tuning->reserved != 0)
+            return fail(RK_ERR_INVALID, "knob out of range");
+        if (tuning->factor_max == 1)
+            return fail(RK_ERR_INVALID, "factor_max must be 0 or >= 2");
+    }
+    struct Scope {
+        explicit Scope(const rk_fft_tuning* t) { rk::t_tuning = t; }
+        ~Scope() { rk::t_tuning = nullptr; }
+    } scope(tuning);
     return rk::plan_create(desc, plan);
 }
 

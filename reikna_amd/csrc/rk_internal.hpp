@@ -110,6 +110,7 @@ struct ChainLimits {
     bool pow2_bluestein;   // Bluestein lengths are powers of two
     bool (*length_ok)(int64_t L);   // pass lengths a kernel exists for (null: any smooth length)
     int64_t strided_single_max;     // single_max of axes that are not contiguous in the input (0: single_max)
+    int64_t first_factor = 0;       // length of the first pass of a multi-pass split (0: the balanced split)
 };
 ChainLimits default_limits();
 

@@ -1,10 +1,18 @@
-"""Measured choice of batch planner knobs.
+"""Measured choice of planner knobs.
 
-For a batch (its items), every candidate :class:`BatchOptions` is planned,
-run on the device and timed with HIP events (median of ``reps`` launches after
-``warmup``); the fastest wins.  Results are cached per device name and item
-list in a JSON file under the run-time cache directory (``RK_CACHE_DIR`` or
-``~/.cache/reikna_amd``), so a batch shape is tuned once per machine.
+Two searches, both timing candidates on the device with HIP events (median
+of ``reps`` executions after ``warmup``) and caching the winner per device
+and problem in a JSON file under the run-time cache directory
+(``RK_CACHE_DIR`` or ``~/.cache/reikna_amd``), so a problem is tuned once
+per machine:
+
+* :func:`autotune_batch` — the batch planner's knobs (single-pass limit,
+  chain pass limit, queue order) for a list of uneven transforms;
+* :func:`autotune_fft` — one transform's plan: the longest chain pass and
+  the first pass length (its radix split), column / row tile widths, the
+  chunk size of chained passes and compiled-in load/store modes
+  (:class:`PlanTuning`).  A candidate whose result departs from the default
+  plan's by more than twice the complex64 error bound is rejected.
 
 No reference counterpart: reikna picks local sizes by halving a limit until a
 kernel compiles (reikna/fft/fft.py:723-756), not by timing.
@@ -20,10 +28,12 @@ from collections.abc import Iterable, Sequence
 from dataclasses import asdict
 from pathlib import Path
 
-from ..runtime import Array, Device, Event, Queue
+from .. import _lib
+from ..runtime import Array, ArrayMetadata, Device, Event, Queue
 from .batch import BatchFFT, BatchItem, BatchOptions
+from .native import NativeFftPlan, PlanTuning
 
-__all__ = ["default_candidates", "autotune_batch"]
+__all__ = ["default_candidates", "autotune_batch", "autotune_fft", "PlanTuning"]
 
 
 def default_candidates() -> list[BatchOptions]:
@@ -35,9 +45,26 @@ def default_candidates() -> list[BatchOptions]:
     return out
 
 
-def _cache_file() -> Path:
+def _cache_file(name: str = "batch_autotune.json") -> Path:
     root = Path(os.environ.get("RK_CACHE_DIR", str(Path.home() / ".cache" / "reikna_amd")))
-    return root / "batch_autotune.json"
+    return root / name
+
+
+def _read_cache(path: Path) -> dict:
+    try:
+        return json.loads(path.read_text()) if path.exists() else {}
+    except (OSError, ValueError):
+        return {}
+
+
+def _write_cache(path: Path, cache: dict) -> None:
+    try:
+        path.parent.mkdir(parents=True, exist_ok=True)
+        tmp = path.with_suffix(".tmp")
+        tmp.write_text(json.dumps(cache, indent=1))
+        os.replace(tmp, path)
+    except OSError:
+        pass
 
 
 def _key(device: Device, items: Sequence[BatchItem]) -> str:
@@ -77,14 +104,9 @@ def autotune_batch(
     items = list(items)
     key = _key(device, items)
     path = _cache_file()
-    cache: dict = {}
-    if use_cache and path.exists():
-        try:
-            cache = json.loads(path.read_text())
-        except (OSError, ValueError):
-            cache = {}
-        if key in cache:
-            return BatchOptions(**cache[key]["options"]), []
+    cache = _read_cache(path) if use_cache else {}
+    if key in cache:
+        return BatchOptions(**cache[key]["options"]), []
     queue = queue or Queue(device)
     n_in = max((it.in_offset + (it.count - 1) * it.in_dist + it.length for it in items if it.count), default=1)
     n_out = max((it.out_offset + (it.count - 1) * it.out_dist + it.length for it in items if it.count), default=1)
@@ -100,11 +122,126 @@ def autotune_batch(
     best = min(results, key=lambda r: r[1])[0]
     if use_cache:
         cache[key] = {"options": asdict(best), "ms": min(r[1] for r in results)}
-        try:
-            path.parent.mkdir(parents=True, exist_ok=True)
-            tmp = path.with_suffix(".tmp")
-            tmp.write_text(json.dumps(cache, indent=1))
-            os.replace(tmp, path)
-        except OSError:
-            pass
+        _write_cache(path, cache)
+    return best, results
+
+
+# ---------------------------------------------------------------------------
+# single-transform plans
+# ---------------------------------------------------------------------------
+
+def _knob_values(native: NativeFftPlan, elements: int) -> dict[str, list]:
+    """Values worth trying for each knob of a plan, from its default pass list."""
+    passes = native.passes
+    by_axis: dict[int, list[int]] = {}
+    for p in passes:
+        by_axis.setdefault(p.axis, []).append(p.L)
+    multi = {a: ls for a, ls in by_axis.items() if len(ls) > 1}
+    knobs: dict[str, list] = {}
+    if multi:
+        knobs["factor_max"] = [0, 128, 512]
+        firsts = set()
+        for ls in multi.values():
+            n = 1
+            for f in ls:
+                n *= f
+            firsts.update(f for f in set(ls) if f != ls[0])
+            firsts.update(f for f in (32, 64, 128, 256) if n % f == 0 and f != ls[0] and n // f > 1)
+        knobs["first_factor"] = [0] + sorted(firsts)[:4]
+    # column-mapped passes have tiles of 16+ instances, row-mapped ones fewer
+    cols = {p.C for p in passes if p.C >= 16}
+    rows = {p.C for p in passes if p.C < 16}
+    if cols:
+        knobs["col_tile"] = [0] + [c for c in (16, 32, 64) if c not in cols]
+    if rows:
+        knobs["row_tile"] = [0] + [c for c in (1, 2, 4, 8) if c not in rows]
+    if native.chunk_rows or 16 * elements > (256 << 20):
+        knobs["chunk_bytes"] = [-1, 0, 64 << 20, 128 << 20, 256 << 20]
+    knobs["fixed_modes"] = [-1, 0]
+    return knobs
+
+
+def _plan_key(device: Device, shape: Sequence[int], axes: Sequence[int]) -> str:
+    return f"{device.name}|{tuple(shape)}|{tuple(axes)}"
+
+
+def autotune_fft(
+    queue: Queue,
+    shape: Sequence[int],
+    axes: Sequence[int] | None = None,
+    reps: int = 10,
+    warmup: int = 10,
+    use_cache: bool = True,
+) -> tuple[PlanTuning, list[tuple[PlanTuning, float]]]:
+    """The fastest :class:`PlanTuning` for a complex64 C-contiguous array of
+    ``shape`` transformed over ``axes``, and every (tuning, ms) measured
+    (empty when the answer came from the cache).
+
+    One coordinate sweep from the planner's own plan: each knob in turn takes
+    each of its values with the others at the best found so far, and a value
+    replaces the current one only when it is more than 2 % faster."""
+    import math
+
+    import numpy
+
+    shape = tuple(int(n) for n in shape)
+    axes = tuple(range(len(shape))) if axes is None else tuple(int(a) for a in axes)
+    key = _plan_key(queue.device, shape, axes)
+    path = _cache_file("fft_autotune.json")
+    cache = _read_cache(path) if use_cache else {}
+    if key in cache:
+        return PlanTuning(**cache[key]["tuning"]), []
+
+    meta = ArrayMetadata(shape, numpy.complex64)
+    elements = math.prod(shape)
+    rng = numpy.random.default_rng(0)
+    host = (rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)).astype(numpy.complex64)
+    inp = Array.from_host(queue, host)
+    out = Array.empty(queue.device, shape, numpy.complex64)
+    ref_out: numpy.ndarray | None = None
+    bound = 2 * 2e-7 * max(1.0, math.log2(math.prod(shape[a] for a in axes)))
+
+    def measure(t: PlanTuning) -> float | None:
+        nonlocal ref_out
+        plan = NativeFftPlan(meta, meta, axes, 0, None if t == PlanTuning() else t)
+        ws = Array.empty(queue.device, (plan.workspace_bytes,), numpy.uint8) if plan.workspace_bytes else None
+        run = lambda: plan.execute(out.ptr, inp.ptr, 0, ws.ptr if ws is not None else None, queue.stream)  # noqa: E731
+        for _ in range(warmup):
+            run()
+        e0, e1 = Event(), Event()
+        times = []
+        for _ in range(reps):
+            e0.record(queue)
+            run()
+            e1.record(queue)
+            e1.synchronize()
+            times.append(e0.elapsed_ms(e1))
+        res = out.get(queue)
+        if ref_out is None:
+            ref_out = res
+        else:
+            den = float(numpy.linalg.norm(ref_out.astype(numpy.complex128)))
+            err = float(numpy.linalg.norm((res - ref_out).astype(numpy.complex128))) / max(den, 1e-30)
+            if not err <= bound:
+                return None
+        return statistics.median(times)
+
+    best = PlanTuning()
+    best_ms = measure(best)
+    results: list[tuple[PlanTuning, float]] = [(best, best_ms)]
+    default_plan = NativeFftPlan(meta, meta, axes, _lib.RK_FFT_DRY_RUN)
+    for knob, values in _knob_values(default_plan, elements).items():
+        for v in values:
+            cand = PlanTuning(**{**asdict(best), knob: v})
+            if cand == best:
+                continue
+            ms = measure(cand)
+            if ms is None:
+                continue
+            results.append((cand, ms))
+            if ms < 0.98 * best_ms:
+                best, best_ms = cand, ms
+    if use_cache:
+        cache[key] = {"tuning": asdict(best), "ms": best_ms}
+        _write_cache(path, cache)
     return best, results

@@ -36,7 +36,7 @@ from ..core.transformation import leaf_name, param_cname
 from ..runtime import Array, ArrayMetadata, DeviceParameters, Queue
 from ..template import render_with_modules
 from ..transformations import copy
-from .native import NativeFftPlan
+from .native import NativeFftPlan, PlanTuning
 
 
 def _meta(arr_t: Any) -> ArrayMetadata:
@@ -75,7 +75,8 @@ class FFT(Computation):
         forward transform, 1 the inverse one.
     """
 
-    def __init__(self, arr_t: Any, axes: Iterable[int] | None = None, *, flags: int = 0):
+    def __init__(self, arr_t: Any, axes: Iterable[int] | None = None, *, flags: int = 0,
+                 tuning: PlanTuning | None = None):
         arr = _meta(arr_t)
         if not dtypes.is_complex(arr.dtype):
             raise ValueError("FFT computation requires array of a complex dtype")
@@ -93,6 +94,7 @@ class FFT(Computation):
         ndim = len(arr.shape)
         self._axes = tuple(range(ndim)) if axes is None else helpers.normalize_axes(ndim, tuple(axes))
         self._flags = flags
+        self._tuning = tuning
         self.native_plan: NativeFftPlan | None = None
 
     def _build_trivial_plan(self, plan_factory: Callable[[], ComputationPlan], args: KernelArguments) -> ComputationPlan:
@@ -120,7 +122,7 @@ class FFT(Computation):
         flags = self._flags | (_lib.RK_FFT_OUT_LAST_ONLY if fused_out else 0)
         if plan.virtual:
             flags |= _lib.RK_FFT_DRY_RUN
-        native = NativeFftPlan(in_meta, out_meta, self._axes, flags)
+        native = NativeFftPlan(in_meta, out_meta, self._axes, flags, self._tuning)
         self.native_plan = native
         ws = plan.temp_array((native.workspace_bytes,), numpy.uint8) if native.workspace_bytes else None
 

@@ -45,6 +45,34 @@ class PassInfo:
     pipe: bool = False          # persistent kernel that loads its next tile while computing one
 
 
+@dataclass(frozen=True)
+class PlanTuning:
+    """Planner knobs of one plan (rk_fft_tuning); the defaults leave every
+    choice to the planner.  ``first_factor`` is the length of the first pass of
+    every multi-pass axis, ``col_tile`` / ``row_tile`` the instances per tile of
+    column- / row-mapped passes, ``chunk_bytes`` the chunk size of chained passes
+    (0: no chunks), ``fixed_modes`` whether kernels have their load/store modes
+    compiled in.  A knob the plan cannot honour is ignored."""
+
+    factor_max: int = 0
+    first_factor: int = 0
+    col_tile: int = 0
+    row_tile: int = 0
+    chunk_bytes: int = -1
+    fixed_modes: int = -1
+
+    def to_c(self) -> _lib.FftTuning:
+        t = _lib.FftTuning()
+        t.factor_max = self.factor_max
+        t.first_factor = self.first_factor
+        t.col_tile = self.col_tile
+        t.row_tile = self.row_tile
+        t.chunk_bytes = self.chunk_bytes
+        t.fixed_modes = self.fixed_modes
+        t.reserved = 0
+        return t
+
+
 def _element_layout(meta: ArrayMetadata) -> tuple[list[int], int]:
     item = meta.dtype.itemsize
     if meta.first_element_offset % item or any(s % item for s in meta.strides):
@@ -61,6 +89,7 @@ class NativeFftPlan:
         out_meta: ArrayMetadata,
         axes: Sequence[int] | None,
         flags: int = 0,
+        tuning: PlanTuning | None = None,
     ):
         if in_meta.shape != out_meta.shape:
             raise ValueError("input and output shapes differ")
@@ -89,8 +118,13 @@ class NativeFftPlan:
         desc.flags = flags
         self.shape = tuple(in_meta.shape)
         self.axes = tuple(axes)
+        self.tuning = tuning
         h = ctypes.c_void_p()
-        _lib.call("rk_fft_plan_create", ctypes.byref(desc), ctypes.byref(h))
+        if tuning is None:
+            _lib.call("rk_fft_plan_create", ctypes.byref(desc), ctypes.byref(h))
+        else:
+            tc = tuning.to_c()
+            _lib.call("rk_fft_plan_create_tuned", ctypes.byref(desc), ctypes.byref(tc), ctypes.byref(h))
         self.handle = h.value
         self._finalizer = weakref.finalize(self, _destroy, self.handle)
         ws = ctypes.c_size_t(0)

@@ -1,0 +1,54 @@
+"""GPU tests of planner knobs and the plan autotuner: every tuned plan stays
+within the complex64 bound (rel-L2 <= 2e-7 * log2 N) of numpy.fft in FP64,
+whichever pass split, tile width, chunking or kernel twin it runs."""
+
+import numpy
+import pytest
+
+import oracle
+from reikna_amd import Array
+from reikna_amd.fft import FFT, PlanTuning
+
+pytestmark = pytest.mark.gpu
+
+
+def tuned_result(queue, data, axes, tuning):
+    arr = Array.from_host(queue, data)
+    out = Array.empty_like(queue.device, arr)
+    fft = FFT(arr, axes=axes, tuning=tuning)
+    fft.compile(queue.device)(queue, out, arr, 0)
+    return out.get(queue), fft.native_plan
+
+
+@pytest.mark.parametrize("knobs", [
+    {"first_factor": 64}, {"factor_max": 1024}, {"factor_max": 64}, {"col_tile": 64}, {"col_tile": 16},
+    {"chunk_bytes": 0}, {"chunk_bytes": 8 << 20}, {"fixed_modes": 0},
+], ids=lambda k: "-".join(f"{a}={b}" for a, b in k.items()))
+def test_tuned_plans_match_oracle(queue, knobs):
+    # 256 MiB: large enough for chained passes to run in chunks
+    shape, axes = (128, 2**18), (1,)
+    data = oracle.test_array(shape, seed=71)
+    res, plan = tuned_result(queue, data, axes, PlanTuning(**knobs))
+    for r in (0, 63, 127):
+        err = oracle.rel_l2(res[r], oracle.fftn(data[r], (0,)))
+        assert err <= oracle.tolerance(shape[1:], (0,)), (knobs, r, err, [(p.L, p.C) for p in plan.passes])
+    if "chunk_bytes" in knobs:
+        assert (plan.chunk_rows > 0) == (knobs["chunk_bytes"] > 0)
+
+
+def test_autotune_fft_picks_a_correct_plan_and_caches(queue, tmp_path, monkeypatch):
+    from reikna_amd.fft.autotune import autotune_fft
+
+    monkeypatch.setenv("RK_CACHE_DIR", str(tmp_path))
+    shape, axes = (8, 3 * 2**16), (1,)
+    best, results = autotune_fft(queue, shape, axes, reps=3, warmup=1)
+    assert results and results[0][0] == PlanTuning()
+    assert all(ms > 0 for _, ms in results)
+    times = dict(results)
+    assert best in times and times[best] <= times[PlanTuning()]
+    data = oracle.test_array(shape, seed=72)
+    res, _ = tuned_result(queue, data, axes, best)
+    assert oracle.rel_l2(res, oracle.fftn(data, (1,), workers=16)) <= oracle.tolerance(shape, axes)
+    # the second call answers from the cache without timing anything
+    again, measured = autotune_fft(queue, shape, axes)
+    assert again == best and measured == []

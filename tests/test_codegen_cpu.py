@@ -257,3 +257,40 @@ def test_plan_chunks_batched_multi_axis(vdev):
     assert n == 2 and len(ps) == 3
     # the transformed leading axis cannot be chunked
     assert chunked((2**14, 4096), (0, 1), vdev)[0] == 0
+
+
+def tuned_plan(shape, axes, dev, **knobs):
+    from reikna_amd.fft import PlanTuning
+
+    fft = FFT(ArrayMetadata(shape, numpy.complex64), axes=axes, tuning=PlanTuning(**knobs))
+    fft.compile(dev)
+    return fft.native_plan
+
+
+def test_plan_tuning_knobs(vdev):
+    # rk_fft_plan_create_tuned: each knob changes the plan it names
+    base = [p.L for p in tuned_plan((64, 2**20), (1,), vdev).passes]
+    assert base == [128, 128, 64]
+    assert [p.L for p in tuned_plan((64, 2**20), (1,), vdev, first_factor=64).passes] == [64, 128, 128]
+    assert [p.L for p in tuned_plan((64, 2**20), (1,), vdev, factor_max=1024).passes] == [1024, 1024]
+    assert [p.L for p in tuned_plan((64, 2**20), (1,), vdev, factor_max=128).passes] == [128, 128, 64]
+    ps = tuned_plan((64, 2**20), (1,), vdev, col_tile=64).passes
+    assert [p.C for p in ps] == [64, 64, 64]
+    assert tuned_plan((64, 2**20), (1,), vdev).chunk_rows == 16
+    assert tuned_plan((64, 2**20), (1,), vdev, chunk_bytes=0).chunk_rows == 0
+    assert tuned_plan((64, 2**20), (1,), vdev, chunk_bytes=64 << 20).chunk_rows == 8
+    assert not any(p.fixed_modes for p in tuned_plan((64, 2**20), (1,), vdev, fixed_modes=0).passes)
+    # knobs the plan cannot honour are ignored: a first factor that does not
+    # divide the axis, a tile wider than 1024 threads
+    assert [p.L for p in tuned_plan((64, 2**20), (1,), vdev, first_factor=3).passes] == base
+    assert [p.C for p in tuned_plan((4, 4096), (1,), vdev, row_tile=64).passes] == \
+        [p.C for p in tuned_plan((4, 4096), (1,), vdev).passes]
+
+
+def test_plan_tuning_out_of_range_refused(vdev):
+    from reikna_amd.fft import PlanTuning
+
+    for knobs in ({"fixed_modes": 2}, {"chunk_bytes": -2}, {"col_tile": -1}, {"factor_max": 1}):
+        with pytest.raises(ValueError, match="knob out of range|factor_max"):
+            tuned_plan((64, 4096), (1,), vdev, **knobs)
+    assert PlanTuning().to_c().reserved == 0
