@@ -306,7 +306,7 @@ typedef struct {
     uint32_t mode;        /* RK_BATCH_MODE_* */
     uint32_t flags;       /* RK_BATCH_DRY_RUN */
     int32_t single_max;   /* longest single-pass length (0: 4096, at most 4096) */
-    int32_t factor_max;   /* longest pass of a multi-pass chain (0: 256, at most 256) */
+    int32_t factor_max;   /* longest pass of a multi-pass chain (0: 512, at most 1024) */
     int32_t blue_max;     /* reserved, 0 (Bluestein runs as two chains in the batch kernel) */
     int32_t order;        /* queue order: 0 critical path (longest chains first, all chains
                              ending together), 1 level by level in item order,

@@ -224,7 +224,7 @@ def catalog_entries() -> list[tuple[int, int, int]]:
 # persistent batch kernel: fixed work-group size, one case per (L, C, V)
 BATCH_THREADS = 256
 BATCH_SINGLE_MAX = 4096
-BATCH_FACTOR_MAX = 256
+BATCH_FACTOR_MAX = 1024
 
 
 def batch_lengths() -> list[int]:

@@ -39,7 +39,7 @@ __all__ = ["default_candidates", "autotune_batch", "autotune_fft", "PlanTuning"]
 def default_candidates() -> list[BatchOptions]:
     out = []
     for single in (1024, 2048, 4096):
-        for factor in (64, 128, 256):
+        for factor in (64, 128, 256, 512, 1024):
             for order in (0, 2):
                 out.append(BatchOptions(single_max=single, factor_max=factor, order=order))
     return out

@@ -65,7 +65,7 @@ class BatchOptions:
 
     mode: str = "queue"      # "queue" (one launch) or "grouped" (one plan per item)
     single_max: int = 0      # longest single-pass length (<= 4096)
-    factor_max: int = 0      # longest pass of a multi-pass chain (<= 256)
+    factor_max: int = 0      # longest pass of a multi-pass chain (<= 1024; 0: 512)
     blue_max: int = 0        # reserved (Bluestein runs as two chains in the batch kernel)
     order: int = 0           # queue order: 0 critical path, 1 levels in item order, 2 levels, largest tiles first
 

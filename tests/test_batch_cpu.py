@@ -23,15 +23,17 @@ def test_single_pass_lengths_one_level(vdev):
 
 
 def test_chain_lengths(vdev):
-    # 8192 -> two passes; 65521 (prime) -> Bluestein over 2^17: two chains of three passes
+    # 8192 -> two passes; 65521 (prime) -> Bluestein over 2^17 = 512 * 256:
+    # two chains of two passes (three each with chain passes of at most 256)
     assert info(vdev, [8192])["levels"] == 2
     assert info(vdev, [49152])["levels"] == 2          # 3 * 2^14 = 192 * 256
-    assert info(vdev, [65521])["levels"] == 6
+    assert info(vdev, [65521])["levels"] == 4
+    assert info(vdev, [65521], factor_max=256)["levels"] == 6
     assert info(vdev, [997])["levels"] == 1            # single-kernel Bluestein over 2048
     assert info(vdev, [5003])["levels"] == 4           # Bluestein over 2^14 = 128 * 128, twice
     i = info(vdev, [8192, 64, 65521])
-    assert i["levels"] == 6 and i["passes"] == 2 + 1 + 6 and i["launches"] == 1
-    assert i["workspace_bytes"] >= 8 * (8192 + 2 * 131072)   # one region for two passes, two for more
+    assert i["levels"] == 4 and i["passes"] == 2 + 1 + 4 and i["launches"] == 1
+    assert i["workspace_bytes"] >= 8 * (8192 + 2 * 131072)   # one region per item, two for Bluestein chains
 
 
 def test_lengths_without_a_case_use_bluestein(vdev):
