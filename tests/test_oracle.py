@@ -142,7 +142,7 @@ def test_radix_array_documented_values():
 
 
 def test_global_radix_info_plan_counts():
-    # pass counts the survey derives for the benchmark sizes (fft.py:89-146)
+    # pass counts reikna's global radix planner gives the benchmark sizes (fft.py:89-146)
     r, r1, r2 = fo.global_radix_info(2**20)
     assert r == [128, 128, 64] and len(r) == 3
     assert fo.global_radix_info(256)[0] == [128, 2]
