@@ -195,6 +195,16 @@ int rk_fft_plan_workspace_size(const rk_fft_plan* plan, size_t* bytes);
 int rk_fft_execute(const rk_fft_plan* plan, void* out, const void* in,
                    int inverse, void* workspace, void* stream);
 
+/* rk_fft_execute for callers that pass buffer sizes: refuses, before any
+   launch, an input or output buffer shorter than the extent its layout in the
+   plan's descriptor reaches (shape, strides, offset), a layout reaching before
+   its buffer's start, or a workspace shorter than rk_fft_plan_workspace_size.
+   Replaces the same call (reikna/core/computation.py:795), where grunnur's
+   typed Array arguments carry their sizes. */
+int rk_fft_execute_checked(const rk_fft_plan* plan, void* out, size_t out_bytes,
+                           const void* in, size_t in_bytes, int inverse,
+                           void* workspace, size_t workspace_bytes, void* stream);
+
 /* Same as rk_fft_execute, with the library's own device scratch array as the
    workspace: it calls no hipMalloc/hipFree and allocates nothing.  No
    counterpart in the reference: for callers that cannot allocate (code under

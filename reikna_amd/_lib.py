@@ -262,6 +262,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "rk_fft_plan_create_tuned": (_I, [ctypes.POINTER(FftDesc), ctypes.POINTER(FftTuning), _PP]),
     "rk_fft_plan_workspace_size": (_I, [_P, ctypes.POINTER(_SZ)]),
     "rk_fft_execute": (_I, [_P, _P, _P, _I, _P, _P]),
+    "rk_fft_execute_checked": (_I, [_P, _P, _SZ, _P, _SZ, _I, _P, _SZ, _P]),
     "rk_fft_execute_scratch": (_I, [_P, _P, _P, _I, _P]),
     "rk_fft_execute_fused": (_I, [_P, _P, _P, _I, _P, _P, _P, _SZ, _P, _SZ]),
     "rk_fft_plan_destroy": (_I, [_P]),

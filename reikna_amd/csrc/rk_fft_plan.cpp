@@ -273,14 +273,30 @@ struct rk_fft_plan {
     bool timing = false;
     std::vector<hipEvent_t> events;   // 2 per pass when timing
     int device = 0;
-    // Chunked execution of a multi-pass single-axis plan: every pass runs on
-    // `chunk_outer` outer-batch indices before the next chunk starts, and the
-    // workspace holds one chunk, so a chunk's intermediate can stay in the
-    // Infinity Cache between the passes.  0 = off.
+    // Chunked execution of a multi-pass plan over an untransformed dim 0:
+    // every pass runs on `chunk_outer` indices of dim 0 before the next chunk
+    // starts, and the workspace holds one chunk, so a chunk's intermediate
+    // can stay in the Infinity Cache between the passes.  0 = off.
     int64_t chunk_outer = 0;
     int64_t n_outer = 1;
-    int64_t ws_outer_stride = 0;   // elements of one outer index in the workspace layout
 };
+
+namespace rk {
+
+// Bytes from a buffer's start that a layout (shape, strides and offset in
+// elements) reaches: one past its farthest element, or -1 when an element
+// would sit before the buffer's start (a negative stride or offset).
+static int64_t layout_extent_bytes(const rk_fft_desc& d, const int64_t* strides, int64_t offset) {
+    int64_t lo = offset, hi = offset;
+    for (int k = 0; k < d.ndim; ++k) {
+        const int64_t span = (d.shape[k] - 1) * strides[k];
+        if (span < 0) lo += span; else hi += span;
+    }
+    if (lo < 0) return -1;
+    return (hi + 1) * (int64_t)sizeof(float2);
+}
+
+}  // namespace rk
 
 namespace rk {
 
@@ -1191,6 +1207,24 @@ extern "C" int rk_fft_plan_create_tuned(const rk_fft_desc* desc, const rk_fft_tu
         ~Scope() { rk::t_tuning = nullptr; }
     } scope(tuning);
     return rk::plan_create(desc, plan);
+}
+
+extern "C" int rk_fft_execute_checked(const rk_fft_plan* plan, void* out, size_t out_bytes, const void* in,
+                                      size_t in_bytes, int inverse, void* workspace, size_t workspace_bytes,
+                                      void* stream) {
+    if (!plan) return fail(RK_ERR_INVALID, "null plan");
+    const int64_t need_in = rk::layout_extent_bytes(plan->desc, plan->desc.in_strides, plan->desc.in_offset);
+    const int64_t need_out = rk::layout_extent_bytes(plan->desc, plan->desc.out_strides, plan->desc.out_offset);
+    if (need_in < 0 || need_out < 0)
+        return fail(RK_ERR_INVALID, "a layout reaches before its buffer's start");
+    if ((int64_t)in_bytes < need_in)
+        return fail(RK_ERR_INVALID, "input buffer of %zu bytes; the plan reads %lld", in_bytes, (long long)need_in);
+    if ((int64_t)out_bytes < need_out)
+        return fail(RK_ERR_INVALID, "output buffer of %zu bytes; the plan writes %lld", out_bytes,
+                    (long long)need_out);
+    if (workspace_bytes < plan->ws_bytes)
+        return fail(RK_ERR_INVALID, "workspace of %zu bytes; the plan needs %zu", workspace_bytes, plan->ws_bytes);
+    return rk::plan_execute(plan, out, in, inverse, workspace, (hipStream_t)stream, nullptr, 0, nullptr, 0);
 }
 
 extern "C" int rk_fft_plan_workspace_size(const rk_fft_plan* plan, size_t* bytes) {

@@ -176,6 +176,16 @@ class NativeFftPlan:
             )
         _lib.check(rc, "rk_fft_execute")
 
+    def execute_checked(self, out_ptr: int, out_bytes: int, in_ptr: int, in_bytes: int, inverse: int,
+                        ws_ptr: int | None, ws_bytes: int, stream: int | None) -> None:
+        """rk_fft_execute_checked: the buffers' sizes are checked against the
+        extents the plan's layouts reach before anything is launched."""
+        rc = _lib.load().rk_fft_execute_checked(
+            ctypes.c_void_p(self.handle), ctypes.c_void_p(out_ptr), int(out_bytes), ctypes.c_void_p(in_ptr),
+            int(in_bytes), int(inverse), ctypes.c_void_p(ws_ptr), int(ws_bytes), ctypes.c_void_p(stream),
+        )
+        _lib.check(rc, "rk_fft_execute_checked")
+
     def set_pass_kernel(self, pass_index: int, function: int) -> None:
         _lib.call("rk_fft_plan_set_pass_kernel", ctypes.c_void_p(self.handle), pass_index, ctypes.c_void_p(function))
 

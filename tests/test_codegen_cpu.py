@@ -294,3 +294,39 @@ def test_plan_tuning_out_of_range_refused(vdev):
         with pytest.raises(ValueError, match="knob out of range|factor_max"):
             tuned_plan((64, 4096), (1,), vdev, **knobs)
     assert PlanTuning().to_c().reserved == 0
+
+
+def test_checked_execute_refuses_short_buffers(vdev):
+    # rk_fft_execute_checked compares the buffer sizes with the extents the
+    # plan's layouts reach before launching anything (here a dry-run plan,
+    # which then refuses to execute)
+    from reikna_amd.runtime import ArrayMetadata as AM
+
+    plan = FFT(AM((8, 4096), numpy.complex64), axes=(1,))
+    plan.compile(vdev)
+    native = plan.native_plan
+    full = 8 * 4096 * 8
+    fake = 1 << 20   # never dereferenced: every check comes first
+    with pytest.raises(ValueError, match="input buffer of"):
+        native.execute_checked(fake, full, fake, full - 8, 0, None, 0, None)
+    with pytest.raises(ValueError, match="output buffer of"):
+        native.execute_checked(fake, full - 1, fake, full, 0, None, 0, None)
+    with pytest.raises(ValueError, match="dry-run plan"):
+        native.execute_checked(fake, full, fake, full, 0, None, 0, None)
+    # a strided view reaches one past its last element from the buffer start
+    view = AM((8, 4096), numpy.complex64, strides=(2 * 4096 * 8, 8), first_element_offset=4096 * 8)
+    pv = FFT(view, axes=(1,))
+    pv.compile(vdev)
+    reach = (4096 + 7 * 2 * 4096 + 4096) * 8
+    with pytest.raises(ValueError, match="input buffer of"):
+        pv.native_plan.execute_checked(fake, reach, fake, reach - 8, 0, None, 0, None)
+    with pytest.raises(ValueError, match="dry-run plan"):
+        pv.native_plan.execute_checked(fake, reach, fake, reach, 0, None, 0, None)
+    # a multi-pass plan needs its workspace
+    big = FFT(AM((64, 2**20), numpy.complex64), axes=(1,))
+    big.compile(vdev)
+    n = 64 * 2**20 * 8
+    ws = big.native_plan.workspace_bytes
+    assert ws > 0
+    with pytest.raises(ValueError, match="workspace of"):
+        big.native_plan.execute_checked(fake, n, fake, n, 0, fake, ws - 1, None)

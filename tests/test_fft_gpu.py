@@ -378,3 +378,23 @@ def test_golden_fixtures(queue, case):
     for inverse, key in ((False, "_fwd"), (True, "_inv")):
         res, _ = run_fft(queue, data, axes, inverse=inverse, inplace=False)
         assert oracle.rel_l2(res, g[case + key]) <= tol, (case, inverse)
+
+
+def test_checked_execute(queue):
+    """rk_fft_execute_checked runs the plan when the buffers hold its layouts'
+    extents and refuses shorter ones before launching anything."""
+    import numpy as np
+
+    data = oracle.test_array((16, 4096), seed=81)
+    inp = Array.from_host(queue, data)
+    out = Array.from_host(queue, np.zeros_like(data))
+    fft = FFT(inp, axes=(1,))
+    fft.compile(queue.device)
+    native = fft.native_plan
+    with pytest.raises(ValueError, match="output buffer of"):
+        native.execute_checked(out.ptr, out.nbytes - 8, inp.ptr, inp.nbytes, 0, None, 0, queue.stream)
+    queue.synchronize()
+    assert not np.any(out.get(queue))          # nothing was launched
+    native.execute_checked(out.ptr, out.nbytes, inp.ptr, inp.nbytes, 0, None, 0, queue.stream)
+    err = oracle.rel_l2(out.get(queue), oracle.fftn(data, (1,)))
+    assert err <= oracle.tolerance((16, 4096), (1,))
