@@ -44,7 +44,7 @@ enum rk_status {
     RK_ERR_BUDGET = -5       /* request exceeds an HBM budget */
 };
 
-enum rk_dtype { RK_C64 = 0 };
+enum rk_dtype { RK_C64 = 0, RK_C128 = 1 };   /* complex64, complex128 (FFT plans only) */
 
 /* ------------------------------------------------------------------ */
 /* Errors                                                              */
@@ -131,7 +131,7 @@ int rk_launch(void* function, const uint32_t grid[3], const uint32_t block[3],
 typedef struct rk_fft_plan rk_fft_plan;
 
 typedef struct {
-    int dtype;                          /* RK_C64 */
+    int dtype;                          /* RK_C64 or RK_C128 */
     int ndim;                           /* 1..RK_MAX_DIMS */
     int64_t shape[RK_MAX_DIMS];
     int64_t in_strides[RK_MAX_DIMS];    /* elements */

@@ -33,6 +33,8 @@ RK_FFT_NO_JIT = 0x4
 RK_FFT_NORM_FORWARD = 0x8
 RK_FFT_OUT_LAST_ONLY = 0x10
 RK_FFT_DRY_RUN = 0x20
+RK_C64 = 0
+RK_C128 = 1
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [

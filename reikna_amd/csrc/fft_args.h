@@ -75,6 +75,7 @@ struct rk_pass_args {
     // multipliers computed on the host, sh = ceil(log2 dim))
     rk_u32 mag_I, mag_NS, mag_H;
     rk_u32 sh_I, sh_NS, sh_H;
+    double out_scale64;      // out_scale in FP64 (complex128 passes)
 };
 
 // Arguments of fused (transformation-carrying) pass kernels: the leaf

@@ -41,45 +41,57 @@
 namespace rkfft {
 
 // ------------------------------------------------------------------------
-// complex arithmetic on float2 (x = re, y = im)
+// complex arithmetic (x = re, y = im) on complex64 (float2) and complex128
+// (double2) values; a pass's element type is its IO policy's value_type
 // ------------------------------------------------------------------------
-RK_HD float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-RK_HD float2 c_sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-RK_HD float2 c_mul(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-RK_HD float2 c_conj(float2 a) { return make_float2(a.x, -a.y); }
-RK_HD float2 c_mul_mi(float2 a) { return make_float2(a.y, -a.x); }   // a * (-i)
-RK_HD float2 c_scale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+template <class X> struct RealOf;
+template <> struct RealOf<float2> { typedef float type; };
+template <> struct RealOf<double2> { typedef double type; };
+template <class X> using real_t = typename RealOf<X>::type;
 
-// Butterfly constants: double literals, rounded once to float.
-#define RK_C8 ((float)0.7071067811865476)     // cos(pi/4)
-#define RK_C16 ((float)0.9238795325112867)    // cos(pi/8)
-#define RK_S16 ((float)0.3826834323650898)    // sin(pi/8)
-#define RK_S3 ((float)0.8660254037844387)     // sin(2pi/3)
-#define RK_C5_1 ((float)0.30901699437494745)  // cos(2pi/5)
-#define RK_C5_2 ((float)-0.8090169943749473)  // cos(4pi/5)
-#define RK_S5_1 ((float)0.9510565162951535)   // sin(2pi/5)
-#define RK_S5_2 ((float)0.5877852522924732)   // sin(4pi/5)
-#define RK_C7_1 ((float)0.6234898018587336)
-#define RK_C7_2 ((float)-0.22252093395631434)
-#define RK_C7_3 ((float)-0.900968867902419)
-#define RK_S7_1 ((float)0.7818314824680298)
-#define RK_S7_2 ((float)0.9749279121818236)
-#define RK_S7_3 ((float)0.43388373911755823)
+template <class X> RK_HD X cx(real_t<X> re, real_t<X> im);
+template <> RK_HD float2 cx<float2>(float re, float im) { return make_float2(re, im); }
+template <> RK_HD double2 cx<double2>(double re, double im) { return make_double2(re, im); }
+
+template <class X> RK_HD X c_add(X a, X b) { return cx<X>(a.x + b.x, a.y + b.y); }
+template <class X> RK_HD X c_sub(X a, X b) { return cx<X>(a.x - b.x, a.y - b.y); }
+template <class X> RK_HD X c_mul(X a, X b) { return cx<X>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+template <class X> RK_HD X c_conj(X a) { return cx<X>(a.x, -a.y); }
+template <class X> RK_HD X c_mul_mi(X a) { return cx<X>(a.y, -a.x); }   // a * (-i)
+template <class X> RK_HD X c_scale(X a, real_t<X> s) { return cx<X>(a.x * s, a.y * s); }
+// cond ? a : b part by part (a select of the whole 16-byte complex128 value
+// is lowered through scratch memory)
+template <class X> RK_HD X c_select(bool cond, X a, X b) { return cx<X>(cond ? a.x : b.x, cond ? a.y : b.y); }
+
+// Butterfly constants: double literals, rounded once to the element's real
+// type where they are used ((F)RK_C8 and so on).
+#define RK_C8 0.7071067811865476      // cos(pi/4)
+#define RK_C16 0.9238795325112867     // cos(pi/8)
+#define RK_S16 0.3826834323650898     // sin(pi/8)
+#define RK_S3 0.8660254037844387      // sin(2pi/3)
+#define RK_C5_1 0.30901699437494745   // cos(2pi/5)
+#define RK_C5_2 -0.8090169943749473   // cos(4pi/5)
+#define RK_S5_1 0.9510565162951535    // sin(2pi/5)
+#define RK_S5_2 0.5877852522924732    // sin(4pi/5)
+#define RK_C7_1 0.6234898018587336
+#define RK_C7_2 -0.22252093395631434
+#define RK_C7_3 -0.900968867902419
+#define RK_S7_1 0.7818314824680298
+#define RK_S7_2 0.9749279121818236
+#define RK_S7_3 0.43388373911755823
 
 // ------------------------------------------------------------------------
 // in-register forward DFTs, y_k = sum_n v_n exp(-2 pi i n k / R), in place
 // ------------------------------------------------------------------------
-RK_HD void dft2(float2& a, float2& b) {
-    float2 t = a;
+template <class X> RK_HD void dft2(X& a, X& b) {
+    X t = a;
     a = c_add(t, b);
     b = c_sub(t, b);
 }
 
-RK_HD void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
-    float2 t0 = c_add(a0, a2), t1 = c_sub(a0, a2);
-    float2 t2 = c_add(a1, a3), t3 = c_mul_mi(c_sub(a1, a3));
+template <class X> RK_HD void dft4(X& a0, X& a1, X& a2, X& a3) {
+    X t0 = c_add(a0, a2), t1 = c_sub(a0, a2);
+    X t2 = c_add(a1, a3), t3 = c_mul_mi(c_sub(a1, a3));
     a0 = c_add(t0, t2);
     a2 = c_sub(t0, t2);
     a1 = c_add(t1, t3);
@@ -87,12 +99,18 @@ RK_HD void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
 }
 
 // multiply by W8^1 = (1 - i)/sqrt2 and W8^3 = (-1 - i)/sqrt2
-RK_HD float2 c_w8_1(float2 a) { return make_float2(RK_C8 * (a.x + a.y), RK_C8 * (a.y - a.x)); }
-RK_HD float2 c_w8_3(float2 a) { return make_float2(RK_C8 * (a.y - a.x), -RK_C8 * (a.x + a.y)); }
+template <class X> RK_HD X c_w8_1(X a) {
+    using F = real_t<X>;
+    return cx<X>((F)RK_C8 * (a.x + a.y), (F)RK_C8 * (a.y - a.x));
+}
+template <class X> RK_HD X c_w8_3(X a) {
+    using F = real_t<X>;
+    return cx<X>((F)RK_C8 * (a.y - a.x), -(F)RK_C8 * (a.x + a.y));
+}
 
-RK_HD void dft8(float2* v) {
-    float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
-    float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+template <class X> RK_HD void dft8(X* v) {
+    X e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    X o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
     dft4(e0, e1, e2, e3);
     dft4(o0, o1, o2, o3);
     o1 = c_w8_1(o1);
@@ -106,14 +124,15 @@ RK_HD void dft8(float2* v) {
 
 // 16 = 4 x 4: DFT4 over n2 for each n1 (x[n1 + 4 n2]), twiddle W16^{n1 k2},
 // DFT4 over n1 for each k2, output y[k2 + 4 k1].
-RK_HD void dft16(float2* v) {
+template <class X> RK_HD void dft16(X* v) {
+    using F = real_t<X>;
     dft4(v[0], v[4], v[8], v[12]);
     dft4(v[1], v[5], v[9], v[13]);
     dft4(v[2], v[6], v[10], v[14]);
     dft4(v[3], v[7], v[11], v[15]);
-    const float2 w1 = make_float2(RK_C16, -RK_S16);
-    const float2 w3 = make_float2(RK_S16, -RK_C16);
-    const float2 w9 = make_float2(-RK_C16, RK_S16);
+    const X w1 = cx<X>((F)RK_C16, -(F)RK_S16);
+    const X w3 = cx<X>((F)RK_S16, -(F)RK_C16);
+    const X w9 = cx<X>(-(F)RK_C16, (F)RK_S16);
     v[5] = c_mul(v[5], w1);
     v[9] = c_w8_1(v[9]);
     v[13] = c_mul(v[13], w3);
@@ -128,7 +147,7 @@ RK_HD void dft16(float2* v) {
     dft4(v[8], v[9], v[10], v[11]);
     dft4(v[12], v[13], v[14], v[15]);
     // v[4 k2 + k1] holds y[k2 + 4 k1]: transpose the 4x4 block
-    float2 t;
+    X t;
     t = v[1]; v[1] = v[4]; v[4] = t;
     t = v[2]; v[2] = v[8]; v[8] = t;
     t = v[3]; v[3] = v[12]; v[12] = t;
@@ -137,26 +156,29 @@ RK_HD void dft16(float2* v) {
     t = v[11]; v[11] = v[14]; v[14] = t;
 }
 
-RK_HD void dft3(float2* v) {
-    float2 t = c_add(v[1], v[2]);
-    float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
-    float2 d = c_sub(v[1], v[2]);
-    float2 s = make_float2(RK_S3 * d.y, -RK_S3 * d.x);   // -i sin(2pi/3) d
+template <class X> RK_HD void dft3(X* v) {
+    using F = real_t<X>;
+    X t = c_add(v[1], v[2]);
+    X m = cx<X>(v[0].x - (F)0.5 * t.x, v[0].y - (F)0.5 * t.y);
+    X d = c_sub(v[1], v[2]);
+    X s = cx<X>((F)RK_S3 * d.y, -(F)RK_S3 * d.x);   // -i sin(2pi/3) d
     v[0] = c_add(v[0], t);
     v[1] = c_add(m, s);
     v[2] = c_sub(m, s);
 }
 
-RK_HD void dft5(float2* v) {
-    float2 t1 = c_add(v[1], v[4]), t2 = c_add(v[2], v[3]);
-    float2 t3 = c_sub(v[1], v[4]), t4 = c_sub(v[2], v[3]);
-    float2 a0 = v[0];
-    float2 b1 = make_float2(a0.x + RK_C5_1 * t1.x + RK_C5_2 * t2.x, a0.y + RK_C5_1 * t1.y + RK_C5_2 * t2.y);
-    float2 b2 = make_float2(a0.x + RK_C5_2 * t1.x + RK_C5_1 * t2.x, a0.y + RK_C5_2 * t1.y + RK_C5_1 * t2.y);
+template <class X> RK_HD void dft5(X* v) {
+    using F = real_t<X>;
+    const F c1 = (F)RK_C5_1, c2 = (F)RK_C5_2, s1 = (F)RK_S5_1, s2 = (F)RK_S5_2;
+    X t1 = c_add(v[1], v[4]), t2 = c_add(v[2], v[3]);
+    X t3 = c_sub(v[1], v[4]), t4 = c_sub(v[2], v[3]);
+    X a0 = v[0];
+    X b1 = cx<X>(a0.x + c1 * t1.x + c2 * t2.x, a0.y + c1 * t1.y + c2 * t2.y);
+    X b2 = cx<X>(a0.x + c2 * t1.x + c1 * t2.x, a0.y + c2 * t1.y + c1 * t2.y);
     // u1 = s1 t3 + s2 t4, u2 = s2 t3 - s1 t4; y1 = b1 - i u1, y2 = b2 - i u2
-    float2 u1 = make_float2(RK_S5_1 * t3.x + RK_S5_2 * t4.x, RK_S5_1 * t3.y + RK_S5_2 * t4.y);
-    float2 u2 = make_float2(RK_S5_2 * t3.x - RK_S5_1 * t4.x, RK_S5_2 * t3.y - RK_S5_1 * t4.y);
-    v[0] = make_float2(a0.x + t1.x + t2.x, a0.y + t1.y + t2.y);
+    X u1 = cx<X>(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+    X u2 = cx<X>(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+    v[0] = cx<X>(a0.x + t1.x + t2.x, a0.y + t1.y + t2.y);
     v[1] = c_add(b1, c_mul_mi(u1));
     v[4] = c_sub(b1, c_mul_mi(u1));
     v[2] = c_add(b2, c_mul_mi(u2));
@@ -165,22 +187,23 @@ RK_HD void dft5(float2* v) {
 
 // Odd prime radix 7: y_k = A_k - i B_k, y_{7-k} = A_k + i B_k with
 // A_k = a0 + sum_n cos(2 pi n k/7)(a_n + a_{7-n}), B_k = sum_n sin(2 pi n k/7)(a_n - a_{7-n}).
-RK_HD void dft7(float2* v) {
-    const float c[4] = {1.0f, RK_C7_1, RK_C7_2, RK_C7_3};
-    const float s[4] = {0.0f, RK_S7_1, RK_S7_2, RK_S7_3};
-    float2 p[4], m[4];
+template <class X> RK_HD void dft7(X* v) {
+    using F = real_t<X>;
+    const F c[4] = {(F)1, (F)RK_C7_1, (F)RK_C7_2, (F)RK_C7_3};
+    const F s[4] = {(F)0, (F)RK_S7_1, (F)RK_S7_2, (F)RK_S7_3};
+    X p[4], m[4];
     for (int n = 1; n <= 3; ++n) {
         p[n] = c_add(v[n], v[7 - n]);
         m[n] = c_sub(v[n], v[7 - n]);
     }
-    float2 a0 = v[0];
-    v[0] = make_float2(a0.x + p[1].x + p[2].x + p[3].x, a0.y + p[1].y + p[2].y + p[3].y);
+    X a0 = v[0];
+    v[0] = cx<X>(a0.x + p[1].x + p[2].x + p[3].x, a0.y + p[1].y + p[2].y + p[3].y);
     for (int k = 1; k <= 3; ++k) {
-        float2 A = a0, B = make_float2(0.0f, 0.0f);
+        X A = a0, B = cx<X>((F)0, (F)0);
         for (int n = 1; n <= 3; ++n) {
             int nk = (n * k) % 7;
-            float cc = nk <= 3 ? c[nk] : c[7 - nk];
-            float ss = nk <= 3 ? s[nk] : -s[7 - nk];
+            F cc = nk <= 3 ? c[nk] : c[7 - nk];
+            F ss = nk <= 3 ? s[nk] : -s[7 - nk];
             A.x += cc * p[n].x; A.y += cc * p[n].y;
             B.x += ss * m[n].x; B.y += ss * m[n].y;
         }
@@ -189,15 +212,16 @@ RK_HD void dft7(float2* v) {
     }
 }
 
-template <int R> RK_HD void dft(float2* v);
-template <> RK_HD void dft<1>(float2*) {}
-template <> RK_HD void dft<2>(float2* v) { dft2(v[0], v[1]); }
-template <> RK_HD void dft<3>(float2* v) { dft3(v); }
-template <> RK_HD void dft<4>(float2* v) { dft4(v[0], v[1], v[2], v[3]); }
-template <> RK_HD void dft<5>(float2* v) { dft5(v); }
-template <> RK_HD void dft<7>(float2* v) { dft7(v); }
-template <> RK_HD void dft<8>(float2* v) { dft8(v); }
-template <> RK_HD void dft<16>(float2* v) { dft16(v); }
+template <int R, class X> RK_HD void dft(X* v) {
+    if constexpr (R == 2) dft2(v[0], v[1]);
+    else if constexpr (R == 3) dft3(v);
+    else if constexpr (R == 4) dft4(v[0], v[1], v[2], v[3]);
+    else if constexpr (R == 5) dft5(v);
+    else if constexpr (R == 7) dft7(v);
+    else if constexpr (R == 8) dft8(v);
+    else if constexpr (R == 16) dft16(v);
+    else static_assert(R == 1, "radix without a butterfly");
+}
 
 // ------------------------------------------------------------------------
 // compile-time radix plan of a length L = 2^a 3^b 5^c 7^d
@@ -268,9 +292,10 @@ RK_HD constexpr int lds_pitch_base(int L) {
     int s = lds_pad(L - 1) + 1;
     return (s % 2 == 0) ? s + 1 : s;
 }
-// A tile's exchanges go through halves (floats) when it needs more than 80 KB
-// at the base pitch.
-RK_HD constexpr bool lds_split(int L, int C) { return C * lds_pitch_base(L) * 8 > 80 * 1024; }
+// A tile's exchanges go through halves (real parts, then imaginary parts)
+// when it needs more than 80 KB at the base pitch; ES is the element size
+// (8 bytes complex64, 16 complex128).
+RK_HD constexpr bool lds_split(int L, int C, int ES = 8) { return C * lds_pitch_base(L) * ES > 80 * 1024; }
 // Row pitch of a tile of C instances of L points (T threads each).  Under the
 // column mapping a wave covers C instances x 64/C threads.  For C a power of
 // two from 2 to 16 (32 when split) and T >= 16, an odd pitch stacks the C
@@ -279,8 +304,10 @@ RK_HD constexpr bool lds_split(int L, int C) { return C * lds_pitch_base(L) * 8 
 // instance its own share of the banks.  (A bank model of every exchange of
 // every catalogue tile: 1.2-3x fewer conflict cycles where this applies,
 // averaged over load/store modes and both mappings; the other tiles keep the
-// odd pitch, which the model favours for them.)
-RK_HD constexpr int lds_pitch_for(int L, int C, int T, bool force_split = false) {
+// odd pitch, which the model favours for them.)  Complex128 tiles keep the
+// odd pitch (the rule is for 64-bit accesses and their 32-bit halves).
+RK_HD constexpr int lds_pitch_for(int L, int C, int T, bool force_split = false, int ES = 8) {
+    if (ES != 8) return lds_pitch_base(L);
     const bool split = force_split || lds_split(L, C);
     const bool pow2 = C > 0 && (C & (C - 1)) == 0;
     if (!pow2 || C < 2 || T < 16 || C > (split ? 32 : 16)) return lds_pitch_base(L);
@@ -333,7 +360,7 @@ __device__ __forceinline__ Inst decompose(const rk_pass_args& a, rk_u64 inst) {
 }
 
 // W_N^e for 0 <= e < N from two FP64 tables; the product is formed in FP64
-// and rounded to float once.
+// and rounded once to the element's real type.
 __device__ __forceinline__ double2 d_mul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -344,7 +371,7 @@ __device__ __forceinline__ double2 twiddle_n64(const rk_pass_args& a, rk_u64 e) 
 
 // The inter-pass twiddles of one thread, W_N^{p*l*tw_mult} for p = p0, p0+dp,
 // p0+2dp, ...: two table lookups for the first one and the step, then an FP64
-// recurrence; every value is rounded to float once.
+// recurrence; every value is rounded once to the element's real type.
 struct TwiddleWalk {
     double2 w, s;
     __device__ __forceinline__ void init(const rk_pass_args& a, rk_u64 l, rk_u64 p0, rk_u64 dp) {
@@ -352,8 +379,8 @@ struct TwiddleWalk {
         w = twiddle_n64(a, p0 * lm);
         s = twiddle_n64(a, dp * lm);
     }
-    __device__ __forceinline__ float2 next() {
-        const float2 r = make_float2((float)w.x, (float)w.y);
+    template <class X> __device__ __forceinline__ X next() {
+        const X r = cx<X>((real_t<X>)w.x, (real_t<X>)w.y);
         w = d_mul(w, s);
         return r;
     }
@@ -364,12 +391,23 @@ struct TwiddleWalk {
 // measured instead: on 64 x 2^20 the 128-point middle pass went from 181 to
 // 226 us, its lanes' table reads scattered over many cache lines per load,
 // while the FP64 walk costs VALU only.)
-template <int NV>
-__device__ __forceinline__ void apply_pass_twiddles(const rk_pass_args& a, rk_u64 l, int p0, int dp, float2* x) {
+template <int NV, class X>
+__device__ __forceinline__ void apply_pass_twiddles(const rk_pass_args& a, rk_u64 l, int p0, int dp, X* x) {
     TwiddleWalk tw;
     tw.init(a, l, (rk_u64)p0, (rk_u64)dp);
 #pragma unroll
-    for (int r = 0; r < NV; ++r) x[r] = c_mul(x[r], tw.next());
+    for (int r = 0; r < NV; ++r) x[r] = c_mul(x[r], tw.template next<X>());
+}
+
+// The table pointers of the pass arguments, as the element type X of a pass
+// (complex64 or complex128 tables; the host builds the ones its plan needs).
+template <class X> __device__ __forceinline__ const X* table(const float2* p) { return reinterpret_cast<const X*>(p); }
+
+// The output scale (1/N on the last pass of an inverse transform) in the
+// element's real type: complex128 passes take the FP64 value.
+template <class X> __device__ __forceinline__ real_t<X> out_scale_of(const rk_pass_args& a) {
+    if constexpr (sizeof(real_t<X>) == 8) return a.out_scale64;
+    else return a.out_scale;
 }
 
 // Plain strided element access: the identity IO policy.
@@ -389,28 +427,32 @@ __device__ __forceinline__ void apply_pass_twiddles(const rk_pass_args& a, rk_u6
 // whose in-place passes re-read what the previous pass just wrote, 22 %.
 // The hint is chosen per side (NTL loads, NTS stores): a chunked chain whose
 // intermediate stays in that cache between passes streams only its first
-// pass's loads and its last pass's stores.
-typedef float rk_v2f __attribute__((ext_vector_type(2)));
-template <bool NTL, bool NTS = NTL> struct GlobalIO {
+// pass's loads and its last pass's stores.  X is the element type
+// (complex64 float2, complex128 double2); every IO policy names it value_type.
+template <class F> using rk_v2 = F __attribute__((ext_vector_type(2)));
+template <bool NTL, bool NTS = NTL, class X = float2> struct GlobalIO {
+    using value_type = X;
     static constexpr bool kBranchFree = true;
-    __device__ __forceinline__ float2 load(const rk_pass_args& a, rk_u64 /*o*/, rk_u64 /*m*/, rk_u64 /*i*/,
-                                           rk_i64 addr) const {
+    __device__ __forceinline__ X load(const rk_pass_args& a, rk_u64 /*o*/, rk_u64 /*m*/, rk_u64 /*i*/,
+                                      rk_i64 addr) const {
+        const X* p = reinterpret_cast<const X*>(a.in) + addr;
         if constexpr (NTL) {
-            const rk_v2f x = __builtin_nontemporal_load(reinterpret_cast<const rk_v2f*>(a.in + addr));
-            return make_float2(x.x, x.y);
+            const rk_v2<real_t<X>> x = __builtin_nontemporal_load(reinterpret_cast<const rk_v2<real_t<X>>*>(p));
+            return cx<X>(x.x, x.y);
         } else {
-            return a.in[addr];
+            return *p;
         }
     }
     __device__ __forceinline__ void store(const rk_pass_args& a, rk_u64 /*o*/, rk_u64 /*k*/, rk_u64 /*i*/,
-                                          rk_i64 addr, float2 v) const {
+                                          rk_i64 addr, X v) const {
+        X* p = reinterpret_cast<X*>(a.out) + addr;
         if constexpr (NTS) {
-            rk_v2f x;
+            rk_v2<real_t<X>> x;
             x.x = v.x;
             x.y = v.y;
-            __builtin_nontemporal_store(x, reinterpret_cast<rk_v2f*>(a.out + addr));
+            __builtin_nontemporal_store(x, reinterpret_cast<rk_v2<real_t<X>>*>(p));
         } else {
-            a.out[addr] = v;
+            *p = v;
         }
     }
 };
@@ -426,6 +468,7 @@ using StreamIO = GlobalIO<true>;
 // with a run-time branch was measured: the second code path raised the
 // batch kernel from 160 to 200 VGPRs and cost more than it saved.)
 struct CoherentIO {
+    using value_type = float2;
     static constexpr bool kBranchFree = true;
     __device__ __forceinline__ float2 load(const rk_pass_args& a, rk_u64 /*o*/, rk_u64 /*m*/, rk_u64 /*i*/,
                                            rk_i64 addr) const {
@@ -481,6 +524,9 @@ struct CoherentIO {
 // many resident work-groups per CU as the tile's LDS allows, so the register
 // budget does not cut the residency to fewer.
 #define RK_VAR_OCC 65536
+// Bit of a kernel's variant number: complex128 elements (FP64 butterflies,
+// twiddles and tables); such kernels come from the run-time compiler only.
+#define RK_VAR_F64 131072
 // MODES template argument of fft_pass_tile: 0 reads the modes from the pass
 // arguments, RK_MODES(load, store) compiles them in.
 #define RK_MODES(load, store) (16 | (int)(load) | ((int)(store) << 2))
@@ -530,10 +576,12 @@ __device__ __forceinline__ Walk out_walk(const rk_pass_args& a, const Inst& ic, 
 // invalid instance (past n_inst) was decomposed as instance 0, so its
 // addresses are valid: it reads them and its results are never stored.
 template <int V, class IO>
-__device__ __forceinline__ float2 load_at(const rk_pass_args& a, const IO& io, const Inst& ic, const Walk& w, int c) {
+__device__ __forceinline__ typename IO::value_type load_at(const rk_pass_args& a, const IO& io, const Inst& ic,
+                                                           const Walk& w, int c) {
+    using X = typename IO::value_type;
     const rk_u64 m = w.m0 + (rk_u64)c * w.mstep;
     const rk_i64 addr = w.a0 + (rk_i64)c * w.astep;
-    const float2 zero = make_float2(0.0f, 0.0f);
+    const X zero = cx<X>((real_t<X>)0, (real_t<X>)0);
     if constexpr (pad_in<V>()) {
         const bool ok = ic.valid && m < a.n_valid_in;
         if constexpr (IO::kBranchFree) {
@@ -542,8 +590,8 @@ __device__ __forceinline__ float2 load_at(const rk_pass_args& a, const IO& io, c
             // shared clamp address would be a hot spot for loads that bypass
             // the caches
             const rk_i64 spare = ic.in_base + (rk_i64)(ic.jpass < a.n_valid_in ? ic.jpass : 0) * a.in_sa;
-            const float2 y = io.load(a, ic.o, m, ic.i, ok ? addr : spare);
-            return ok ? y : zero;
+            const X y = io.load(a, ic.o, m, ic.i, ok ? addr : spare);
+            return c_select(ok, y, zero);
         } else {
             return ok ? io.load(a, ic.o, m, ic.i, addr) : zero;
         }
@@ -560,8 +608,8 @@ __device__ __forceinline__ float2 load_at(const rk_pass_args& a, const IO& io, c
 // (c_i = cpos(i)): conjugation for the inverse direction (a uniform branch),
 // then the Bluestein chirp (compile-time).  The inter-pass twiddle (PRE_TWIDDLE)
 // is applied by the caller's TwiddleWalk.
-template <int V, int NV, class FC>
-__device__ __forceinline__ void pre_ops(const rk_pass_args& a, const Inst& ic, const Walk& w, float2* x, FC cpos) {
+template <int V, int NV, class X, class FC>
+__device__ __forceinline__ void pre_ops(const rk_pass_args& a, const Inst& ic, const Walk& w, X* x, FC cpos) {
     if (a.in_sign < 0.0f) {
 #pragma unroll
         for (int i = 0; i < NV; ++i) x[i].y = -x[i].y;
@@ -571,22 +619,24 @@ __device__ __forceinline__ void pre_ops(const rk_pass_args& a, const Inst& ic, c
         for (int i = 0; i < NV; ++i) {
             const rk_u64 m = w.m0 + (rk_u64)cpos(i) * w.mstep;
             const bool ok = ic.valid && m < a.n_valid_in;
-            x[i] = c_mul(x[i], a.chirp[ok ? m : 0]);
+            x[i] = c_mul(x[i], table<X>(a.chirp)[ok ? m : 0]);
         }
     }
 }
 
 // The output-side operations of one value at axis position kabs: the
 // compile-time post-op, then conjugation and scale for the direction.
-template <int V>
-__device__ __forceinline__ float2 post_op(const rk_pass_args& a, rk_u64 kabs, float2 y, bool adjust) {
+template <class X> __device__ __forceinline__ X adjust_out(const rk_pass_args& a, X y) {
+    y.y *= (real_t<X>)a.out_sign;
+    return c_scale(y, out_scale_of<X>(a));
+}
+
+template <int V, class X>
+__device__ __forceinline__ X post_op(const rk_pass_args& a, rk_u64 kabs, X y, bool adjust) {
     constexpr int POST = Variant<V>::POST;
-    if constexpr (POST == RK_VAR_POST_KMUL_CONJ) y = c_conj(c_mul(y, a.bspec[kabs]));
-    if constexpr (POST == RK_VAR_POST_CONJ_CHIRP) y = c_mul(c_conj(y), a.chirp[kabs]);
-    if (adjust) {
-        y.y *= a.out_sign;
-        y = c_scale(y, a.out_scale);
-    }
+    if constexpr (POST == RK_VAR_POST_KMUL_CONJ) y = c_conj(c_mul(y, table<X>(a.bspec)[kabs]));
+    if constexpr (POST == RK_VAR_POST_CONJ_CHIRP) y = c_mul(c_conj(y), table<X>(a.chirp)[kabs]);
+    if (adjust) y = adjust_out(a, y);
     return y;
 }
 
@@ -594,7 +644,7 @@ __device__ __forceinline__ float2 post_op(const rk_pass_args& a, rk_u64 kabs, fl
 // instances); a padded side skips positions at or beyond n_valid_out.
 template <int V, class IO>
 __device__ __forceinline__ void store_at(const rk_pass_args& a, const IO& io, const Inst& ic, const Walk& w, int c,
-                                         float2 y, bool adjust) {
+                                         typename IO::value_type y, bool adjust) {
     const rk_u64 kabs = w.m0 + (rk_u64)c * w.mstep;
     if constexpr (pad_out<V>())
         if (kabs >= a.n_valid_out) return;
@@ -627,10 +677,11 @@ template <bool WAVE> __device__ __forceinline__ void exchange_sync() {
     }
 }
 
-template <bool SPLIT, int NW, int NR, bool WAVE = false, class FW, class FR>
-__device__ __forceinline__ void lds_exchange(void* base, const float2* w, float2* r, FW wpos, FR rpos) {
+template <bool SPLIT, int NW, int NR, bool WAVE = false, class X, class FW, class FR>
+__device__ __forceinline__ void lds_exchange(void* base, const X* w, X* r, FW wpos, FR rpos) {
+    using F = real_t<X>;
     if constexpr (!SPLIT) {
-        float2* lds = reinterpret_cast<float2*>(base);
+        X* lds = reinterpret_cast<X*>(base);
         exchange_sync<WAVE>();
 #pragma unroll
         for (int i = 0; i < NW; ++i) {
@@ -644,8 +695,8 @@ __device__ __forceinline__ void lds_exchange(void* base, const float2* w, float2
             if (p >= 0) r[i] = lds[p];
         }
     } else {
-        float* f = reinterpret_cast<float*>(base);
-        float wy[NW], rx[NR];
+        F* f = reinterpret_cast<F*>(base);
+        F wy[NW], rx[NR];
 #pragma unroll
         for (int i = 0; i < NW; ++i) wy[i] = w[i].y;
         exchange_sync<WAVE>();
@@ -658,7 +709,7 @@ __device__ __forceinline__ void lds_exchange(void* base, const float2* w, float2
 #pragma unroll
         for (int i = 0; i < NR; ++i) {
             const int p = rpos(i);
-            rx[i] = p >= 0 ? f[p] : 0.0f;
+            rx[i] = p >= 0 ? f[p] : (F)0;
         }
         exchange_sync<WAVE>();
 #pragma unroll
@@ -670,7 +721,7 @@ __device__ __forceinline__ void lds_exchange(void* base, const float2* w, float2
 #pragma unroll
         for (int i = 0; i < NR; ++i) {
             const int p = rpos(i);
-            if (p >= 0) r[i] = make_float2(rx[i], f[p]);
+            if (p >= 0) r[i] = cx<X>(rx[i], f[p]);
         }
     }
 }
@@ -707,8 +758,8 @@ __device__ __forceinline__ int stage_out_pos(int t, int i) {  // output position
     return (j / NS) * NS * R + (j % NS) + r * NS;
 }
 
-template <int L, int S, bool SPLIT, int VM, bool WAVE = false>
-__device__ __forceinline__ void run_stages(float2 (&v)[VM], void* my, int t, const float2* twl) {
+template <int L, int S, bool SPLIT, int VM, bool WAVE = false, class X>
+__device__ __forceinline__ void run_stages(X (&v)[VM], void* my, int t, const X* twl) {
     using P = Plan<L>;
     constexpr int R = P::radix(S);
     constexpr int NS = P::ns(S);
@@ -762,18 +813,19 @@ __device__ __forceinline__ rk_u64 xcd_tile(const rk_pass_args& a) {
 // LDS bytes of a tile and whether its exchanges are split into halves.
 // HALF forces the split exchange on a tile that would fit whole (half the LDS,
 // twice the resident work-groups).
-template <int L, int C, bool HALF = false> __host__ __device__ constexpr bool tile_split() {
-    return HALF || lds_split(L, C);
+template <int L, int C, bool HALF = false, int ES = 8> __host__ __device__ constexpr bool tile_split() {
+    return HALF || lds_split(L, C, ES);
 }
-template <int L, int C, bool HALF = false> __host__ __device__ constexpr int tile_pitch() {
-    return lds_pitch_for(L, C, Plan<L>::T, HALF);
+template <int L, int C, bool HALF = false, int ES = 8> __host__ __device__ constexpr int tile_pitch() {
+    return lds_pitch_for(L, C, Plan<L>::T, HALF, ES);
 }
-template <int L, int C, bool HALF = false> __host__ __device__ constexpr int tile_lds_bytes() {
-    return C * tile_pitch<L, C, HALF>() * (tile_split<L, C, HALF>() ? 4 : 8);
+template <int L, int C, bool HALF = false, int ES = 8> __host__ __device__ constexpr int tile_lds_bytes() {
+    return C * tile_pitch<L, C, HALF, ES>() * (tile_split<L, C, HALF, ES>() ? ES / 2 : ES);
 }
 
 // One tile (instances tile*C ... tile*C + C - 1) computed by threads
-// tid = 0 ... C*T - 1 with `lds` of at least tile_lds_bytes<L, C>() bytes.
+// tid = 0 ... C*T - 1 with `lds` of at least tile_lds_bytes<L, C, HALF, ES>()
+// bytes (ES = sizeof the IO policy's element type).
 // Every value a thread computes and every address it writes is a function of
 // (a, tile, tid) alone, so a caller with more threads than C*T may give the
 // surplus threads tid = C*T - 1: they repeat that thread's work, writing the
@@ -781,21 +833,24 @@ template <int L, int C, bool HALF = false> __host__ __device__ constexpr int til
 // With PRE, the tile's DIRECT loads were issued earlier (raw values in
 // `pre`, see direct_raw_loads): the tile starts from them instead of loading.
 template <int L, int C, int V, class IO, int MODES = 0, bool WAVE = false, bool PRE = false, bool HALF = false>
-__device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& io, float2* lds, rk_u64 tile, int tid,
-                                              const float2* pre = nullptr) {
+__device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& io, typename IO::value_type* lds,
+                                              rk_u64 tile, int tid, const typename IO::value_type* pre = nullptr) {
+    using X = typename IO::value_type;
+    using F = real_t<X>;
+    constexpr int ES = (int)sizeof(X);
     using P = Plan<L>;
     static_assert(!WAVE || (64 % P::T == 0), "wave-local stages need whole instances per wave");
     constexpr int T = P::T;
     constexpr int NT = C * T;
     constexpr int R0 = P::R0;
-    constexpr int LP = tile_pitch<L, C, HALF>();
+    constexpr int LP = tile_pitch<L, C, HALF, ES>();
     constexpr int VM = P::vmax();
     constexpr int SL = P::NSTAGE - 1;
     constexpr int RL = P::radix(SL);
     constexpr int QL = P::q(SL);
     constexpr bool BLUE = Variant<V>::BLUE;
     constexpr bool TWID = Variant<V>::PRE == RK_VAR_PRE_TWIDDLE;
-    constexpr bool SPLIT = tile_split<L, C, HALF>();
+    constexpr bool SPLIT = tile_split<L, C, HALF, ES>();
     const rk_u64 inst0 = tile * (rk_u64)C;
     const rk_u32 load_mode = MODES ? (rk_u32)(MODES & 3) : a.load_mode;
     const rk_u32 store_mode = MODES ? (rk_u32)((MODES >> 2) & 3) : a.store_mode;
@@ -810,8 +865,8 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
     const int cs = colmap ? (tid / T) : (tid % C);
     const int ps = colmap ? (tid % T) : (tid / C);
 
-    float2 v[VM];
-    float2 g[R0];   // staging registers (global side of a staged load/store)
+    X v[VM];
+    X g[R0];   // staging registers (global side of a staged load/store)
 
     // ---- stage-0 inputs: element p = t + r*T of instance c ----
     const Inst ic = decompose<L>(a, inst0 + c);
@@ -834,11 +889,11 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
         for (int it = 0; it < R0; ++it) {
             const int e = tid + it * NT;
             const int ce = e / L, p = e - ce * L;
-            float2 x = make_float2(0.0f, 0.0f);
+            X x = cx<X>((F)0, (F)0);
             const bool ok = inst0 + ce < a.n_inst;
             if constexpr (IO::kBranchFree) {
-                const float2 y = io.load(a, inst0 + ce, p, 0, ok ? base + e : base);
-                x = ok ? y : x;
+                const X y = io.load(a, inst0 + ce, p, 0, ok ? base + e : base);
+                x = c_select(ok, y, x);
             } else {
                 if (ok) x = io.load(a, inst0 + ce, p, 0, base + e);
             }
@@ -865,11 +920,12 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
     }
 
     // LDS of this thread's instance (the stages index it relative to the instance)
-    void* my = SPLIT ? (void*)(reinterpret_cast<float*>(lds) + cbase) : (void*)(lds + cbase);
+    void* my = SPLIT ? (void*)(reinterpret_cast<F*>(lds) + cbase) : (void*)(lds + cbase);
     auto last_out = [&](int i) {
         return stage_valid<L, SL>(t, i / RL) ? cbase + lds_pad(stage_out_pos<L, SL>(t, i)) : -1;
     };
-    run_stages<L, 0, SPLIT, VM, WAVE>(v, my, t, a.twl);
+    const X* twl = table<X>(a.twl);
+    run_stages<L, 0, SPLIT, VM, WAVE>(v, my, t, twl);
 
     if constexpr (BLUE) {
         // A = FFT(chirp * x): multiply by the kernel spectrum, conjugate, and
@@ -878,7 +934,7 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
         for (int i = 0; i < QL * RL; ++i)
             if (stage_valid<L, SL>(t, i / RL)) {
                 const int k = stage_out_pos<L, SL>(t, i);
-                v[i] = c_conj(c_mul(v[i], a.bspec[k]));
+                v[i] = c_conj(c_mul(v[i], table<X>(a.bspec)[k]));
             }
         lds_exchange<SPLIT, QL * RL, R0, WAVE>(lds, v, v, last_out, own_pos);
         // the second transform and the stores recompute what depends on the
@@ -887,7 +943,7 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
         // in registers across both transforms (the 2048-point tile: 162 -> 124
         // VGPRs, four waves per SIMD instead of two)
         asm volatile("" : "+v"(t));
-        run_stages<L, 0, SPLIT, VM, WAVE>(v, my, t, a.twl);
+        run_stages<L, 0, SPLIT, VM, WAVE>(v, my, t, table<X>(a.twl));
     }
 
     // ---- store: output element k = stage_out_pos of the last stage ----
@@ -914,11 +970,8 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
             const int e = tid + it * NT;
             const int ce = e / L, k = e - ce * L;
             if (inst0 + ce < a.n_inst) {
-                float2 y = g[it];
-                if (adjust) {
-                    y.y *= a.out_sign;
-                    y = c_scale(y, a.out_scale);
-                }
+                X y = g[it];
+                if (adjust) y = adjust_out(a, y);
                 io.store(a, inst0 + ce, k, 0, base + e, y);
             }
         }
@@ -941,8 +994,10 @@ template <int L, int C> __host__ __device__ constexpr int pass_threads() { retur
 // work-group's own LDS.
 template <int L, int C, int V, class IO, int MODES = 0, bool WAVE = false, bool HALF = false>
 __device__ __forceinline__ void fft_pass_body(const rk_pass_args& a, const IO& io) {
-    constexpr int LP = tile_pitch<L, C, HALF>();
-    __shared__ float2 lds[tile_split<L, C, HALF>() ? (C * LP + 1) / 2 : C * LP];
+    using X = typename IO::value_type;
+    constexpr int ES = (int)sizeof(X);
+    constexpr int LP = tile_pitch<L, C, HALF, ES>();
+    __shared__ X lds[tile_split<L, C, HALF, ES>() ? (C * LP + 1) / 2 : C * LP];
     const rk_u64 tile = xcd_tile(a) + a.tile_base;
     if (tile >= a.n_tiles) return;
     fft_pass_tile<L, C, V, IO, MODES, WAVE, false, HALF>(a, io, lds, tile, (int)threadIdx.x);
@@ -951,7 +1006,7 @@ __device__ __forceinline__ void fft_pass_body(const rk_pass_args& a, const IO& i
 // The raw DIRECT loads of one tile (what fft_pass_tile<..., PRE> starts from).
 template <int L, int C, int V, class IO, bool WAVE>
 __device__ __forceinline__ void direct_raw_loads(const rk_pass_args& a, const IO& io, rk_u64 tile, int tid,
-                                                 float2* raw) {
+                                                 typename IO::value_type* raw) {
     using P = Plan<L>;
     constexpr int T = P::T;
     const bool colmap = WAVE ? false : a.cmap != 0;
@@ -970,14 +1025,16 @@ __device__ __forceinline__ void direct_raw_loads(const rk_pass_args& a, const IO
 // The host launches about one work-group per resident slot.
 template <int L, int C, int V, class IO, int MODES, bool WAVE, bool HALF>
 __device__ __forceinline__ void fft_pass_body_pipe(const rk_pass_args& a, const IO& io) {
-    constexpr int LP = tile_pitch<L, C, HALF>();
+    using X = typename IO::value_type;
+    constexpr int ES = (int)sizeof(X);
+    constexpr int LP = tile_pitch<L, C, HALF, ES>();
     constexpr int R0 = Plan<L>::R0;
-    __shared__ float2 lds[tile_split<L, C, HALF>() ? (C * LP + 1) / 2 : C * LP];
+    __shared__ X lds[tile_split<L, C, HALF, ES>() ? (C * LP + 1) / 2 : C * LP];
     const int tid0 = (int)threadIdx.x;
     const rk_u64 G = gridDim.x;
     rk_u64 tile = a.tile_base + blockIdx.x;
     if (tile >= a.n_tiles) return;
-    float2 nxt[R0];
+    X nxt[R0];
     direct_raw_loads<L, C, V, IO, WAVE>(a, io, tile, tid0, nxt);
     for (;;) {
         // everything derived from the thread index (stage twiddle indices and
@@ -986,7 +1043,7 @@ __device__ __forceinline__ void fft_pass_body_pipe(const rk_pass_args& a, const 
         // empty asm makes the index opaque to the optimiser each iteration
         int tid = tid0;
         asm volatile("" : "+v"(tid));
-        float2 cur[R0];
+        X cur[R0];
 #pragma unroll
         for (int r = 0; r < R0; ++r) cur[r] = nxt[r];
         const rk_u64 next = tile + G;
@@ -997,14 +1054,20 @@ __device__ __forceinline__ void fft_pass_body_pipe(const rk_pass_args& a, const 
     }
 }
 
+// The element type of a variant number: complex128 with RK_VAR_F64.
+template <bool F64> struct ElemOf { typedef float2 type; };
+template <> struct ElemOf<true> { typedef double2 type; };
+template <int V> using elem_t = typename ElemOf<(V & RK_VAR_F64) != 0>::type;
+
 // The pass kernel of a catalogue variant number V: the low seven bits are the
 // compile-time Variant (pre-op, post-op, Bluestein), the higher bits choose the
 // IO policy (RK_VAR_STREAM, RK_VAR_NT_LOAD, RK_VAR_NT_STORE), compiled-in
-// load/store modes (RK_VAR_MODE_*) and wave-local stages (RK_VAR_WAVE).  The
-// ahead-of-time catalogue and the run-time compiler instantiate this same
-// entry.
+// load/store modes (RK_VAR_MODE_*), wave-local stages (RK_VAR_WAVE) and the
+// element type (RK_VAR_F64).  The ahead-of-time catalogue and the run-time
+// compiler instantiate this same entry.
 template <int L, int C, int V>
 __device__ __forceinline__ void fft_pass_entry(const rk_pass_args& a) {
+    using X = elem_t<V>;
     constexpr int VB = V & (RK_VAR_STREAM - 1);
     constexpr int M = (V & RK_VAR_MODE_DD) ? RK_MODES(RK_MODE_DIRECT, RK_MODE_DIRECT)
                     : (V & RK_VAR_MODE_DF) ? RK_MODES(RK_MODE_DIRECT, RK_MODE_FLAT)
@@ -1014,9 +1077,9 @@ __device__ __forceinline__ void fft_pass_entry(const rk_pass_args& a) {
     constexpr bool WAVE = (V & RK_VAR_WAVE) != 0;
     constexpr bool HALF = (V & RK_VAR_HALF) != 0;
     if constexpr ((V & RK_VAR_PIPE) != 0)
-        fft_pass_body_pipe<L, C, VB, GlobalIO<NTL, NTS>, M, WAVE, HALF>(a, GlobalIO<NTL, NTS>{});
+        fft_pass_body_pipe<L, C, VB, GlobalIO<NTL, NTS, X>, M, WAVE, HALF>(a, GlobalIO<NTL, NTS, X>{});
     else
-        fft_pass_body<L, C, VB, GlobalIO<NTL, NTS>, M, WAVE, HALF>(a, GlobalIO<NTL, NTS>{});
+        fft_pass_body<L, C, VB, GlobalIO<NTL, NTS, X>, M, WAVE, HALF>(a, GlobalIO<NTL, NTS, X>{});
 }
 
 // Minimum waves per SIMD of a pass kernel's launch bounds: three for the
@@ -1030,7 +1093,8 @@ __device__ __forceinline__ void fft_pass_entry(const rk_pass_args& a) {
 template <int L, int C, int V> __host__ __device__ constexpr int pass_min_waves() {
     if ((V & (RK_VAR_PIPE | RK_VAR_OCC)) != 0) {
         const int waves = (pass_threads<L, C>() + 63) / 64;
-        const int slots = (V & RK_VAR_OCC) ? (160 * 1024) / tile_lds_bytes<L, C, (V & RK_VAR_HALF) != 0>() : 1;
+        const int slots = (V & RK_VAR_OCC) ? (160 * 1024) / tile_lds_bytes<L, C, (V & RK_VAR_HALF) != 0,
+                                                                           (int)sizeof(elem_t<V>)>() : 1;
         const int w = (slots * waves) / 4;
         return w < 1 ? 1 : (w > 8 ? 8 : w);
     }

@@ -390,6 +390,7 @@ static int create_queue(const rk_batch_item* items, int n_items, rk_batch_plan* 
         if (it.length == 1) {   // identity: a strided copy launched beside the queue
             rk::CopyArgs c;
             memset(&c, 0, sizeof(c));
+            c.words = 1;
             c.ndim = 1;
             c.shape[0] = it.count;
             c.in_str[0] = it.in_dist;

@@ -1,5 +1,5 @@
 // rk_copy.hip — strided N-d copy used by trivial FFT plans (every transformed
-// axis has length 1).  Mirrors FFT._build_trivial_plan (reikna/fft/fft.py:647-662),
+// axis has length 1), of elements of `words` 8-byte words.  Mirrors FFT._build_trivial_plan (reikna/fft/fft.py:647-662),
 // which copies through PureParallel so that attached transformations still run.
 #include "rk_internal.hpp"
 
@@ -17,7 +17,7 @@ __global__ void __launch_bounds__(256) rk_copy_kernel(CopyArgs a) {
             io += (rk_i64)k * a.in_str[d];
             oo += (rk_i64)k * a.out_str[d];
         }
-        a.out[oo] = a.in[io];
+        for (int w = 0; w < a.words; ++w) a.out[oo * a.words + w] = a.in[io * a.words + w];
     }
 }
 

@@ -32,9 +32,10 @@ using rk::fail;
 namespace rk {
 
 // Knobs of the plan being built on this thread (rk_fft_plan_create_tuned;
-// null: the planner's own choices).  Set for the duration of one plan
-// creation only.
+// null: the planner's own choices), and its element bytes (8 complex64, 16
+// complex128).  Set for the duration of one plan creation only.
 static thread_local const rk_fft_tuning* t_tuning = nullptr;
+static thread_local int t_elem_bytes = 8;
 
 // Longest single-pass DFT of a smooth length, and longest single-kernel
 // Bluestein convolution (RK_SINGLE_MAX overrides both, at most 16384).  Above
@@ -56,7 +57,13 @@ static int64_t env_single_max() {
     return v;
 }
 static int64_t single_max() { return env_single_max() ? env_single_max() : 4096; }
-static int64_t blue_single_max() { return env_single_max() ? env_single_max() : 8192; }
+// Complex128 tiles hold twice the registers: their single-kernel Bluestein
+// stops at M = 2048 (256 threads, 196 VGPRs), longer convolutions run as
+// chains.
+static int64_t blue_single_max() {
+    if (env_single_max()) return env_single_max();
+    return t_elem_bytes == 16 ? 2048 : 8192;
+}
 #define kSingleMax (single_max())
 
 // Longest DFT of one pass in a multi-pass chain.  Column-style passes need
@@ -141,8 +148,9 @@ int threads_per_instance(int L) {
 
 int lds_pitch_host(int L) { return rkfft::lds_pitch_base(L); }
 
-int tile_lds_bytes_host(int L, int C) {
-    return C * rkfft::lds_pitch_for(L, C, threads_per_instance(L)) * (rkfft::lds_split(L, C) ? 4 : 8);
+int tile_lds_bytes_host(int L, int C, int es) {
+    const int pitch = rkfft::lds_pitch_for(L, C, threads_per_instance(L), false, es);
+    return C * pitch * (rkfft::lds_split(L, C, es) ? es / 2 : es);
 }
 
 // Tuning overrides of the tile widths for experiments (kernels outside the
@@ -153,7 +161,8 @@ int tile_lds_bytes_host(int L, int C) {
 // exceed 1024 threads or 159 KB of LDS.
 static int tile_if_fits(int c, int L) {
     if (c <= 0) return 0;
-    if ((int64_t)c * threads_per_instance(L) > 1024 || (int64_t)c * lds_pitch_host(L) * 8 > 159 * 1024) return 0;
+    if ((int64_t)c * threads_per_instance(L) > 1024 || (int64_t)c * lds_pitch_host(L) * t_elem_bytes > 159 * 1024)
+        return 0;
     return c;
 }
 
@@ -165,7 +174,7 @@ static int forced_tile(const char* name, int L) {
 static int default_tile_c_row(int L) {
     int T = threads_per_instance(L);
     int C = std::max(1, 256 / T);
-    while (C > 1 && (int64_t)C * lds_pitch_host(L) * 8 > 64 * 1024) C /= 2;
+    while (C > 1 && (int64_t)C * lds_pitch_host(L) * t_elem_bytes > 64 * 1024) C /= 2;
     return C;
 }
 
@@ -186,7 +195,7 @@ int tile_c_col(int L) {
         if (int f = forced_tile("RK_COL_TILE", L)) return f;
     int T = threads_per_instance(L);
     int C = std::max(16, (default_tile_c_row(L) / 16) * 16);
-    while (C > 1 && ((int64_t)C * T > 1024 || (int64_t)C * lds_pitch_host(L) * 8 > 159 * 1024)) C /= 2;
+    while (C > 1 && ((int64_t)C * T > 1024 || (int64_t)C * lds_pitch_host(L) * t_elem_bytes > 159 * 1024)) C /= 2;
     // tuning override: an upper bound on the column tile width (kernels outside
     // the catalog come from the run-time compiler)
     static const int cap = [] {
@@ -269,6 +278,7 @@ struct rk_fft_plan {
     size_t ws_bytes = 0, ws1_offset = 0;
     double norm = 1.0;
     bool trivial = false;
+    int es = 8;   // element bytes: 8 complex64, 16 complex128
     rk::CopyArgs copy;
     bool timing = false;
     std::vector<hipEvent_t> events;   // 2 per pass when timing
@@ -286,14 +296,14 @@ namespace rk {
 // Bytes from a buffer's start that a layout (shape, strides and offset in
 // elements) reaches: one past its farthest element, or -1 when an element
 // would sit before the buffer's start (a negative stride or offset).
-static int64_t layout_extent_bytes(const rk_fft_desc& d, const int64_t* strides, int64_t offset) {
+static int64_t layout_extent_bytes(const rk_fft_desc& d, const int64_t* strides, int64_t offset, int es) {
     int64_t lo = offset, hi = offset;
     for (int k = 0; k < d.ndim; ++k) {
         const int64_t span = (d.shape[k] - 1) * strides[k];
         if (span < 0) lo += span; else hi += span;
     }
     if (lo < 0) return -1;
-    return (hi + 1) * (int64_t)sizeof(float2);
+    return (hi + 1) * (int64_t)es;
 }
 
 }  // namespace rk
@@ -491,6 +501,7 @@ void set_div_magic(rk_pass_args& a) {
 int describe_pass(const AxisPass& ap, const Layout& ls, const Layout& ld, int axis, TileWidthFn tile_width,
                   PassPlan& pp) {
     memset(&pp.args, 0, sizeof(pp.args));
+    pp.es = t_elem_bytes;
     pp.axis = axis;
     pp.L = ap.L;
     pp.kind = ap.blue_single ? 1 : 0;
@@ -533,6 +544,7 @@ int describe_pass(const AxisPass& ap, const Layout& ls, const Layout& ld, int ax
     a.in_sign = 1.0f;
     a.out_sign = 1.0f;
     a.out_scale = 1.0f;
+    a.out_scale64 = 1.0;
 
     // global-memory walk of each side
     const int64_t L = ap.L;
@@ -623,7 +635,7 @@ static double stream_min_bytes() {
 static bool stream_pass(const PassPlan& pp) {
     if (pp.variant != VAR_PLAIN && pp.variant != VAR_TWIDDLE) return false;
     if (stream_min_bytes() <= 0) return false;
-    const double bytes = 8.0 * (double)pp.args.n_inst * (double)pp.L * (pp.src == pp.dst ? 1.0 : 2.0);
+    const double bytes = (double)pp.es * (double)pp.args.n_inst * (double)pp.L * (pp.src == pp.dst ? 1.0 : 2.0);
     return bytes > stream_min_bytes();
 }
 
@@ -663,6 +675,7 @@ static int io_variant_bits(const PassPlan& pp) {
     if (pp.pipe) v |= VAR_PIPE;
     if (pp.half) v |= VAR_HALF;
     if (pp.occ) v |= VAR_OCC;
+    if (pp.es == 16) v |= VAR_F64;
     return v;
 }
 
@@ -700,7 +713,7 @@ static int find_kernel(PassPlan& pp, uint32_t flags) {
     pp.fixed_modes = mbit != 0;
     if (flags & RK_FFT_DRY_RUN) {   // describe the kernel the run-time compiler would build
         pp.threads = pp.C * threads_per_instance(pp.L);
-        pp.lds_bytes = tile_lds_bytes_host(pp.L, pp.C);
+        pp.lds_bytes = tile_lds_bytes_host(pp.L, pp.C, pp.es);
         return RK_OK;
     }
     return jit_pass_kernel(pp.L, pp.C, want | mbit, &pp.jit, &pp.threads, &pp.lds_bytes);
@@ -787,15 +800,25 @@ static void choose_wave(PassPlan& pp, uint32_t flags) {
     pp.wave = true;
 }
 
+// A table of complex values in the pass's element type: FP64 values rounded
+// once to complex64, or kept as complex128.
+static int64_t add_table(TableBuilder& tb, const PassPlan& pp, const std::string& key, const std::vector<cd>& v) {
+    if (pp.es == 16) {
+        std::vector<double2> d(v.size());
+        for (size_t k = 0; k < v.size(); ++k) d[k] = make_double2(v[k].real(), v[k].imag());
+        return tb.add(key + "/c128", d.data(), d.size() * sizeof(double2));
+    }
+    std::vector<float2> f(v.size());
+    for (size_t k = 0; k < v.size(); ++k) f[k] = make_float2((float)v[k].real(), (float)v[k].imag());
+    return tb.add(key, f.data(), f.size() * sizeof(float2));
+}
+
 int build_tables(TableBuilder& tb, PassPlan& pp, const AxisPass& ap, int64_t nreal) {
     // in-pass stage twiddles W_L^e
     {
-        std::vector<float2> w(pp.L);
-        for (int e = 0; e < pp.L; ++e) {
-            cd z = unit_root(e, pp.L);
-            w[e] = make_float2((float)z.real(), (float)z.imag());
-        }
-        pp.twl_off = tb.add("twl" + std::to_string(pp.L), w.data(), w.size() * sizeof(float2));
+        std::vector<cd> w(pp.L);
+        for (int e = 0; e < pp.L; ++e) w[e] = unit_root(e, pp.L);
+        pp.twl_off = add_table(tb, pp, "twl" + std::to_string(pp.L), w);
     }
     if (ap.pre_ops & RK_PRE_TWIDDLE) {
         int64_t N = ap.Nax;
@@ -820,15 +843,14 @@ int build_tables(TableBuilder& tb, PassPlan& pp, const AxisPass& ap, int64_t nre
     const bool needs_chirp = (ap.pre_ops & RK_PRE_CHIRP) || (ap.post_ops & RK_POST_CHIRP);
     if (needs_chirp) {
         // exp(-i pi (n^2 mod 2N) / N): the phase index is an exact integer
-        std::vector<float2> ch(nreal);
+        std::vector<cd> ch(nreal);
         const int64_t two_n = 2 * nreal;
         for (int64_t k = 0; k < nreal; ++k) {
             const unsigned __int128 sq = (unsigned __int128)k * (unsigned __int128)k;
             const int64_t ph = (int64_t)(sq % (unsigned __int128)two_n);
-            cd z = unit_root(ph, two_n);   // exp(-2 pi i ph / 2N) = exp(-i pi ph / N)
-            ch[k] = make_float2((float)z.real(), (float)z.imag());
+            ch[k] = unit_root(ph, two_n);   // exp(-2 pi i ph / 2N) = exp(-i pi ph / N)
         }
-        pp.chirp_off = tb.add("chirp" + std::to_string(nreal), ch.data(), ch.size() * sizeof(float2));
+        pp.chirp_off = add_table(tb, pp, "chirp" + std::to_string(nreal), ch);
     }
     const bool needs_bspec = ap.blue_single || (ap.post_ops & RK_POST_KMUL_CONJ);
     if (needs_bspec) {
@@ -842,13 +864,8 @@ int build_tables(TableBuilder& tb, PassPlan& pp, const AxisPass& ap, int64_t nre
             if (k > 0) b[m - k] = c;
         }
         fft_host(b);
-        std::vector<float2> bs(m);
-        for (int64_t k = 0; k < m; ++k) {
-            cd z = b[k] / (double)m;
-            bs[k] = make_float2((float)z.real(), (float)z.imag());
-        }
-        pp.bspec_off = tb.add("bspec" + std::to_string(nreal) + "_" + std::to_string(m), bs.data(),
-                              bs.size() * sizeof(float2));
+        for (int64_t k = 0; k < m; ++k) b[k] /= (double)m;
+        pp.bspec_off = add_table(tb, pp, "bspec" + std::to_string(nreal) + "_" + std::to_string(m), b);
     }
     return RK_OK;
 }
@@ -914,7 +931,14 @@ int route_passes(const rk_fft_desc& d, const std::vector<int>& work, const Layou
 static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
     if (!d || !out) return fail(RK_ERR_INVALID, "rk_fft_plan_create: null argument");
     *out = nullptr;
-    if (d->dtype != RK_C64) return fail(RK_ERR_UNSUPPORTED, "only complex64 is supported");
+    if (d->dtype != RK_C64 && d->dtype != RK_C128)
+        return fail(RK_ERR_UNSUPPORTED, "dtype must be RK_C64 or RK_C128");
+    // the element size every sizing decision of this plan uses (tile widths,
+    // tables, workspace, chunks), for this thread and this plan only
+    struct ElemScope {
+        explicit ElemScope(int es) { t_elem_bytes = es; }
+        ~ElemScope() { t_elem_bytes = 8; }
+    } elem_scope(d->dtype == RK_C128 ? 16 : 8);
     if (d->ndim < 1 || d->ndim > RK_MAX_DIMS) return fail(RK_ERR_INVALID, "ndim must be in [1, %d]", RK_MAX_DIMS);
     for (int k = 0; k < d->ndim; ++k)
         if (d->shape[k] < 1) return fail(RK_ERR_INVALID, "shape[%d] = %lld must be >= 1", k, (long long)d->shape[k]);
@@ -934,6 +958,7 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
 
     std::unique_ptr<rk_fft_plan> plan(new rk_fft_plan());
     plan->desc = *d;
+    plan->es = t_elem_bytes;
     hipGetDevice(&plan->device);
 
     Layout lin, lout;
@@ -959,6 +984,7 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
         plan->trivial = true;
         CopyArgs& c = plan->copy;
         memset(&c, 0, sizeof(c));
+        c.words = plan->es / 8;
         c.ndim = d->ndim;
         c.total = 1;
         for (int k = 0; k < d->ndim; ++k) {
@@ -1016,9 +1042,9 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
         const int64_t chunk_bytes = (t_tuning && t_tuning->chunk_bytes >= 0) ? t_tuning->chunk_bytes
                                     : chunk_env >= 0 ? chunk_env
                                     : (two_ws ? (int64_t)128 << 20 : (int64_t)256 << 20);
-        const double pass_bytes = 16.0 * (double)plan->passes.front().args.n_inst * plan->passes.front().L;
+        const double pass_bytes = 2.0 * t_elem_bytes * (double)plan->passes.front().args.n_inst * plan->passes.front().L;
         if (ok && chunk_bytes > 0 && stream_min_bytes() > 0 && pass_bytes > stream_min_bytes()) {
-            const int64_t chunk = std::max<int64_t>(1, chunk_bytes / (per_row_elems * (int64_t)sizeof(float2)));
+            const int64_t chunk = std::max<int64_t>(1, chunk_bytes / (per_row_elems * (int64_t)t_elem_bytes));
             if (chunk < B) {
                 plan->chunk_outer = chunk;
                 plan->n_outer = B;
@@ -1040,7 +1066,7 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
             pp.nt_load = pp.nt_store = stream_pass(pp);
         }
         pp.pipe = pipe_enabled() && pp.args.load_mode == RK_MODE_DIRECT && pp.loops.empty();
-        pp.half = half_enabled() && (int64_t)pp.C * lds_pitch_host(pp.L) * 8 <= 80 * 1024;
+        pp.half = half_enabled() && (int64_t)pp.C * lds_pitch_host(pp.L) * pp.es <= 80 * 1024;
         pp.occ = occ_min_length() > 0 && pp.L >= occ_min_length();
         rc = find_kernel(pp, d->flags);
         if (rc != RK_OK) return rc;
@@ -1051,7 +1077,7 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
     }
     plan->ws_bytes = 0;
     if (ws_elems) {
-        const size_t one = ((ws_elems * sizeof(float2)) + 255) & ~size_t(255);
+        const size_t one = ((ws_elems * (size_t)t_elem_bytes) + 255) & ~size_t(255);
         plan->ws1_offset = one;
         plan->ws_bytes = two_ws ? 2 * one : one;
         // a plan whose passes use WS1 without ever needing two at once still
@@ -1147,10 +1173,14 @@ static int plan_execute(const rk_fft_plan* plan, void* out, const void* in, int 
             if (pp.is_first) a.in_sign = -1.0f;
             if (pp.is_last) {
                 a.out_sign = -1.0f;
-                if (!norm_forward) a.out_scale = (float)plan->norm;
+                if (!norm_forward) {
+                    a.out_scale = (float)plan->norm;
+                    a.out_scale64 = plan->norm;
+                }
             }
         } else if (norm_forward && pp.is_last) {
             a.out_scale = (float)plan->norm;
+            a.out_scale64 = plan->norm;
         }
     }
     if (!plan->chunk_outer) {
@@ -1213,8 +1243,9 @@ extern "C" int rk_fft_execute_checked(const rk_fft_plan* plan, void* out, size_t
                                       size_t in_bytes, int inverse, void* workspace, size_t workspace_bytes,
                                       void* stream) {
     if (!plan) return fail(RK_ERR_INVALID, "null plan");
-    const int64_t need_in = rk::layout_extent_bytes(plan->desc, plan->desc.in_strides, plan->desc.in_offset);
-    const int64_t need_out = rk::layout_extent_bytes(plan->desc, plan->desc.out_strides, plan->desc.out_offset);
+    const int64_t need_in = rk::layout_extent_bytes(plan->desc, plan->desc.in_strides, plan->desc.in_offset, plan->es);
+    const int64_t need_out =
+        rk::layout_extent_bytes(plan->desc, plan->desc.out_strides, plan->desc.out_offset, plan->es);
     if (need_in < 0 || need_out < 0)
         return fail(RK_ERR_INVALID, "a layout reaches before its buffer's start");
     if ((int64_t)in_bytes < need_in)

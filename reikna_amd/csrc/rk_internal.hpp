@@ -57,6 +57,7 @@ enum {
     VAR_PIPE = 1 << 14,    // catalogue entries: persistent software-pipelined body (RK_VAR_PIPE)
     VAR_HALF = 1 << 15,    // catalogue entries: exchanges through re/im halves (RK_VAR_HALF)
     VAR_OCC = 1 << 16,     // catalogue entries: registers bounded for the LDS-allowed residency (RK_VAR_OCC)
+    VAR_F64 = 1 << 17,     // complex128 elements (RK_VAR_F64; run-time compiled only)
 };
 
 // All ahead-of-time kernels compiled into this library.
@@ -66,7 +67,7 @@ const std::vector<KernelEntry>& catalog();
 int tile_c_row(int L);
 int tile_c_col(int L);
 int lds_pitch_host(int L);            // base pitch (tile width choices)
-int tile_lds_bytes_host(int L, int C);   // LDS of a tile (fft_device.hpp tile_lds_bytes)
+int tile_lds_bytes_host(int L, int C, int es = 8);   // LDS of a tile (fft_device.hpp tile_lds_bytes)
 int threads_per_instance(int L);
 bool is_smooth(int64_t n);
 
@@ -131,6 +132,7 @@ struct PassPlan {
     bool pipe = false;              // persistent software-pipelined kernel: one launch per tile range
     bool half = false;              // exchanges through re/im halves (half the LDS)
     bool occ = false;               // registers bounded for the LDS-allowed residency
+    int es = 8;                     // element bytes: 8 complex64, 16 complex128
     bool fixed_modes = false;       // the kernel has this pass's load/store modes compiled in
     uint64_t grid = 0;
     int axis = 0;
@@ -205,6 +207,7 @@ inline int batch_lds_bytes(int flavour) { return flavour ? batch_blue::lds_bytes
 struct CopyArgs {
     const float2* in;
     float2* out;
+    int words = 1;   // 8-byte words per element (1 complex64, 2 complex128)
     int ndim;
     rk_i64 shape[RK_MAX_DIMS];
     rk_i64 in_str[RK_MAX_DIMS];

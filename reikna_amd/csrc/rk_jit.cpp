@@ -55,7 +55,8 @@ int jit_pass_kernel(int L, int C, int variant, hipFunction_t* fn, int* threads, 
         RK_HIP(hipModuleLoadData(&k.module, code.data()));
         RK_HIP(hipModuleGetFunction(&k.fn, k.module, "rk_jit_pass"));
         k.threads = C * T;
-        k.lds = (variant & VAR_HALF) ? tile_lds_bytes_host(L, C) / 2 : tile_lds_bytes_host(L, C);
+        const int es = (variant & VAR_F64) ? 16 : 8;
+        k.lds = (variant & VAR_HALF) ? tile_lds_bytes_host(L, C, es) / 2 : tile_lds_bytes_host(L, C, es);
         it = cache.emplace(key, k).first;
     }
     *fn = it->second.fn;

@@ -80,9 +80,10 @@ class FFT(Computation):
         arr = _meta(arr_t)
         if not dtypes.is_complex(arr.dtype):
             raise ValueError("FFT computation requires array of a complex dtype")
-        if arr.dtype != numpy.complex64:
-            # the kernels compute in complex64 (FP32 butterflies, FP64-derived tables)
-            raise NotImplementedError(f"FFT of {arr.dtype}: this implementation computes complex64 transforms")
+        if arr.dtype not in (numpy.complex64, numpy.complex128):
+            # the kernels compute complex64 (FP32 butterflies, FP64-derived
+            # tables) and complex128 (FP64 throughout)
+            raise NotImplementedError(f"FFT of {arr.dtype}: this implementation computes complex64 and complex128 transforms")
         Computation.__init__(
             self,
             [
@@ -197,10 +198,12 @@ class _FusedFft:
         def bind(src: str, leaves: list[Parameter]) -> str:
             return "".join(f"        auto {leaf_name(p.name)} = {src}.{leaf_name(p.name)};\n" for p in leaves)
 
+        xtype = "double2" if self.native.dtype == numpy.complex128 else "float2"
+
         def render() -> str:
             parts = [struct("RkLoadArgs", self.load_leaves), struct("RkStoreArgs", self.store_leaves)]
-            load_body = "        return a.in[addr];\n"
-            store_body = "        a.out[addr] = v;\n"
+            load_body = "        return reinterpret_cast<const value_type*>(a.in)[addr];\n"
+            store_body = "        reinterpret_cast<value_type*>(a.out)[addr] = v;\n"
             if in_obj is not None:
                 mod = in_obj.load_combined_idx(slices)
                 load_body = bind("la", self.load_leaves) + f"        return {mod}((long long)o, (long long)m, (long long)i);\n"
@@ -209,14 +212,15 @@ class _FusedFft:
                 store_body = bind("sa", self.store_leaves) + f"        {mod}((long long)o, (long long)k, (long long)i, v);\n"
             parts.append(
                 "struct RkFusedIO {\n"
+                f"    using value_type = {xtype};\n"
                 "    static constexpr bool kBranchFree = false;\n"
 
                 "    const RkLoadArgs& la;\n"
                 "    const RkStoreArgs& sa;\n"
-                "    __device__ __forceinline__ float2 load(const rk_pass_args& a, rk_u64 o, rk_u64 m, rk_u64 i,"
+                f"    __device__ __forceinline__ {xtype} load(const rk_pass_args& a, rk_u64 o, rk_u64 m, rk_u64 i,"
                 " rk_i64 addr) const {\n" + load_body + "    }\n"
                 "    __device__ __forceinline__ void store(const rk_pass_args& a, rk_u64 o, rk_u64 k, rk_u64 i,"
-                " rk_i64 addr, float2 v) const {\n" + store_body + "    }\n"
+                f" rk_i64 addr, {xtype} v) const {{\n" + store_body + "    }\n"
                 "};\n"
             )
             # the pass's load/store modes compiled in (fft_device.hpp RK_MODES)

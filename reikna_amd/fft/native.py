@@ -93,15 +93,16 @@ class NativeFftPlan:
     ):
         if in_meta.shape != out_meta.shape:
             raise ValueError("input and output shapes differ")
-        if in_meta.dtype != numpy.complex64 or out_meta.dtype != numpy.complex64:
+        if in_meta.dtype != out_meta.dtype or in_meta.dtype not in (numpy.complex64, numpy.complex128):
             raise NotImplementedError(
-                f"the gfx950 FFT kernels compute in complex64; got {in_meta.dtype} -> {out_meta.dtype}"
+                f"the gfx950 FFT kernels compute complex64 or complex128; got {in_meta.dtype} -> {out_meta.dtype}"
             )
         ndim = len(in_meta.shape)
         if ndim > _lib.RK_MAX_DIMS:
             raise ValueError(f"at most {_lib.RK_MAX_DIMS} dimensions are supported")
         desc = _lib.FftDesc()
-        desc.dtype = 0
+        desc.dtype = _lib.RK_C128 if in_meta.dtype == numpy.complex128 else _lib.RK_C64
+        self.dtype = in_meta.dtype
         desc.ndim = ndim
         in_str, in_off = _element_layout(in_meta)
         out_str, out_off = _element_layout(out_meta)

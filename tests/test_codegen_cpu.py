@@ -217,12 +217,49 @@ def test_plan_xcd_groups_for_narrow_column_tiles(vdev):
     assert [(p.L, p.C, p.load_mode) for p in ps] == [(4096, 4, "direct")]
 
 
-def test_complex128_refused():
-    # the kernels compute complex64; a complex128 array is refused, not misread
-    with pytest.raises(NotImplementedError, match="complex64"):
-        FFT(ArrayMetadata((4, 64), numpy.complex128))
+def test_other_dtypes_refused():
+    # the kernels compute complex64 and complex128; a wider complex type is
+    # refused, not misread, and a real array is not an FFT argument
+    if numpy.dtype(numpy.clongdouble).itemsize > 16:
+        with pytest.raises(NotImplementedError, match="complex128"):
+            FFT(ArrayMetadata((4, 64), numpy.clongdouble))
     with pytest.raises(ValueError, match="complex dtype"):
         FFT(ArrayMetadata((4, 64), numpy.float64))
+
+
+def test_complex128_plans(vdev):
+    # complex128 plans: the same pass chains with twice the bytes per element,
+    # so tiles of half the instances in the same LDS, twice the workspace,
+    # chunks of half the rows, and fused transformations in double2
+    for shape, axes in (((64, 2**20), (1,)), ((4, 65521), (1,)), ((32, 64, 48), (0, 1, 2)), ((1024, 1024), (1,))):
+        p64 = FFT(ArrayMetadata(shape, numpy.complex64), axes=axes)
+        p128 = FFT(ArrayMetadata(shape, numpy.complex128), axes=axes)
+        p64.compile(vdev)
+        p128.compile(vdev)
+        a, b = p64.native_plan, p128.native_plan
+        assert [p.L for p in a.passes] == [p.L for p in b.passes]
+        assert all(q.C == max(1, p.C // 2) for p, q in zip(a.passes, b.passes))
+        assert all(q.lds_bytes <= 160 * 1024 for q in b.passes)
+        assert b.chunk_rows * 2 == a.chunk_rows
+    big = FFT(ArrayMetadata((4, 65521), numpy.complex128), axes=(1,))
+    big.compile(vdev)
+    small = FFT(ArrayMetadata((4, 65521), numpy.complex64), axes=(1,))
+    small.compile(vdev)
+    assert big.native_plan.workspace_bytes == 2 * small.native_plan.workspace_bytes
+    # single-kernel Bluestein up to M = 2048 (longer convolutions as chains)
+    blue = FFT(ArrayMetadata((8, 997), numpy.complex128), axes=(1,))
+    blue.compile(vdev)
+    assert [p.kind for p in blue.native_plan.passes] == [1]
+    chains = FFT(ArrayMetadata((8, 2003), numpy.complex128), axes=(1,))
+    chains.compile(vdev)
+    assert len(chains.native_plan.passes) >= 2
+    arr = ArrayMetadata((64, 4096), numpy.complex128)
+    fft = FFT(arr, axes=(1,))
+    scale = mul_param(arr, numpy.float64)
+    fft.parameter.input.connect(scale, scale.output, input_prime=scale.input, param=scale.param)
+    div = div_const(arr, 8.0)
+    fft.parameter.output.connect(div, div.input, output_prime=div.output)
+    fft.compile(vdev)
 
 
 def test_array_arguments_checked_against_the_compiled_layout():
