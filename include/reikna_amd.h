@@ -18,7 +18,7 @@
  *   - Every function returns an int status: 0 (RK_OK) on success, a negative
  *     RK_ERR_* code otherwise; rk_last_error() then returns a thread-local
  *     message.  Nothing here throws across the boundary.
- *   - Strides and offsets are in ELEMENTS (complex64 = 8 bytes), not bytes.
+ *   - Strides and offsets are in ELEMENTS (complex64 = 8 bytes, complex128 = 16), not bytes.
  *   - Streams are hipStream_t passed as void* (NULL = the default stream).
  *   - Plans are immutable after creation: concurrent rk_fft_execute calls on
  *     different streams with different workspaces are safe.
