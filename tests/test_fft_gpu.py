@@ -398,3 +398,30 @@ def test_checked_execute(queue):
     native.execute_checked(out.ptr, out.nbytes, inp.ptr, inp.nbytes, 0, None, 0, queue.stream)
     err = oracle.rel_l2(out.get(queue), oracle.fftn(data, (1,)))
     assert err <= oracle.tolerance((16, 4096), (1,))
+
+
+def test_one_plan_on_two_streams(queue):
+    """A plan is immutable: the same native plan runs on two streams at once,
+    each call with its own workspace, and both results are correct."""
+    from reikna_amd import ArrayMetadata
+    from reikna_amd.fft.native import NativeFftPlan
+    from reikna_amd.runtime import Queue
+
+    shape = (16, 2**16)
+    meta = ArrayMetadata(shape, numpy.complex64)
+    plan = NativeFftPlan(meta, meta, (1,))
+    assert plan.workspace_bytes > 0
+    q2 = Queue(queue.device)
+    datas = [oracle.test_array(shape, seed=s) for s in (101, 102)]
+    ins = [Array.from_host(queue, d) for d in datas]
+    outs = [Array.empty(queue.device, shape, numpy.complex64) for _ in range(2)]
+    wss = [Array.empty(queue.device, (plan.workspace_bytes,), numpy.uint8) for _ in range(2)]
+    queue.synchronize()
+    for _ in range(8):
+        for k, q in enumerate((queue, q2)):
+            plan.execute(outs[k].ptr, ins[k].ptr, 0, wss[k].ptr, q.stream)
+    queue.synchronize()
+    q2.synchronize()
+    for k in range(2):
+        err = oracle.rel_l2(outs[k].get(queue), oracle.fftn(datas[k], (1,)))
+        assert err <= oracle.tolerance(shape, (1,)), (k, err)
