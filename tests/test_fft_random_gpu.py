@@ -93,3 +93,26 @@ def test_mismatched_output_array_refused(queue):
     small = Array.empty(queue.device, view.shape, numpy.complex64)
     with pytest.raises(ValueError, match="strides"):
         fft(queue, small, inp, 0)
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_configuration_complex128(queue, seed):
+    """The same random configurations in complex128 (run-time compiled FP64
+    kernels), at rel-L2 <= 5e-16 * log2 N against numpy.fft."""
+    rng = numpy.random.default_rng(3000 + seed)
+    shape, axes = random_case(rng)
+    if math.prod(shape) > 2**20:
+        shape = tuple(min(n, 1024) for n in shape)
+    data = oracle.test_array(shape, seed=seed).astype(numpy.complex128)
+    inverse = bool(rng.integers(2))
+    arr = Array.from_host(queue, data)
+    out = Array.empty_like(queue.device, arr)
+    FFT(arr, axes=axes).compile(queue.device)(queue, out, arr, int(inverse))
+    res = out.get(queue)
+    if all(shape[a] == 1 for a in axes):
+        assert numpy.array_equal(res, data)
+        return
+    ref = numpy.fft.ifftn(data, axes=axes) if inverse else numpy.fft.fftn(data, axes=axes)
+    tol = 5e-16 * max(1.0, math.log2(math.prod(shape[a] for a in axes)))
+    err = oracle.rel_l2(res, ref)
+    assert err <= tol, f"{shape} axes {axes} inverse={inverse}: rel-L2 {err:.3g} > {tol:.3g}"
