@@ -128,7 +128,7 @@ def cpu_baseline(shape, axes, data, budget_s: float = 12.0, fwd_inv: bool = Fals
         "unit": "GFLOP/s",
         "cores": workers,
         "kind": "port",
-        "sample": f"{lib}, {sample.shape} complex64 over axes {tuple(axes)} ({sample_rows} of {shape[0]} rows), "
+        "sample": f"{lib}, {sample.shape} {sample.dtype} over axes {tuple(axes)} ({sample_rows} of {shape[0]} rows), "
         f"min of {len(times)} runs; host {cpu_model}, os.cpu_count()={os.cpu_count()}",
         "numpy_1_thread": {
             "value": round(one_flops / min(one_times) / 1e9, 2),
@@ -146,6 +146,8 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="c2b", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--dtype", choices=("c64", "c128"), default="c64",
+                    help="single transforms: complex64 (the configured workloads) or complex128")
     ap.add_argument("--autotune", action="store_true",
                     help="pick planner knobs by measurement (c4: batch knobs; single transforms: plan knobs)")
     ap.add_argument("--requests", type=int, default=1000, help="c5: requests per rate point")
@@ -179,6 +181,8 @@ def main(argv=None) -> int:
     queue = Queue(dev)
     print(f"[bench] {desc} on {dev.name} ({dev.arch}, {dev.params.compute_units} CUs)", file=sys.stderr, flush=True)
     data = oracle.test_array(shape, seed=seed, dist=dist)
+    if args.dtype == "c128":
+        data = data.astype(numpy.complex128)
     arr_in = Array.from_host(queue, data)
     arr_out = Array.empty_like(dev, arr_in)
     tuning, tuned = None, []
@@ -260,6 +264,8 @@ def main(argv=None) -> int:
         rows = [0, shape[0] - 1]
         err = max(oracle.rel_l2(res[r], oracle.fftn(data[r], [a - 1 for a in axes])) for r in rows)
     tol = oracle.tolerance(shape, axes)
+    if args.dtype == "c128":   # the complex128 bound of the GPU tests
+        tol = 5e-16 * max(1.0, math.log2(math.prod(shape[a] for a in axes)))
     if fwd_inv:   # the inverse of the forward result gives the input back
         err = max(err, oracle.rel_l2(arr_back.get(queue), data) / 2)
 
@@ -269,11 +275,11 @@ def main(argv=None) -> int:
     gflops = flops / (ms_per_step / 1e3) / 1e9
     # algorithmic bytes: one read and one write of every element per
     # transform, however many passes the plan makes
-    alg_bytes = 16.0 * elems
+    alg_bytes = 2.0 * data.dtype.itemsize * elems
     transform_gbs = directions * alg_bytes / (ms_per_step / 1e3) / 1e9
     # the same bytes over the dominant pass's time: what one pass streams
     pass_gbs = alg_bytes / (pass_ms[dom] / 1e3) / 1e9
-    traffic = MEASURED_TRAFFIC.get(args.workload)
+    traffic = MEASURED_TRAFFIC.get(args.workload) if args.dtype == "c64" else None
     out = {
         "metric": METRIC,
         "value": round(gflops, 1),
@@ -285,7 +291,7 @@ def main(argv=None) -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64",
+        "dtype": args.dtype,
         "data": "synthetic",
         "config": {
             "workload": desc,
@@ -308,7 +314,7 @@ def main(argv=None) -> int:
             "traffic": traffic[0] if traffic else None,
             "traffic_per": "step (all passes)",
             "traffic_source": traffic[1] if traffic else None,
-            "kernel": f"whole transform: {len(passes)} pass(es), 16 B per element per transform over ms_per_step",
+            "kernel": f"whole transform: {len(passes)} pass(es), {2 * data.dtype.itemsize} B per element per transform over ms_per_step",
             "dominant_pass": f"pass {dom} (L={passes[dom].L}, C={passes[dom].C}), {pass_ms[dom] * 1e3:.1f} us avg",
             "pass_GBs": round(pass_gbs, 1),
             "pass_frac": round(pass_gbs / HBM_PEAK_GBS, 4),
