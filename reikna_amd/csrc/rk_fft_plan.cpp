@@ -221,36 +221,94 @@ static cd unit_root(int64_t e, int64_t n) {
     return cd(std::cos(ang), std::sin(ang));
 }
 
-// Forward DFT in FP64 of a smooth length (recursive mixed radix), used only
-// for the Bluestein kernel spectrum.
+// One Stockham pass of radix R over n points (NS = product of the earlier
+// radices): butterfly j = g*NS + k reads src[j + r*n/R], multiplies by
+// W_{NS R}^{r k}, and writes its outputs to dst[g*NS*R + k + q*NS].
+template <int R, class Root>
+static void host_stage(const cd* src, cd* dst, int64_t n, int64_t ns, const Root& root) {
+    // products written out (std::complex's operator* checks for infinities)
+    auto mul = [](cd a, cd b) {
+        return cd(a.real() * b.real() - a.imag() * b.imag(), a.real() * b.imag() + a.imag() * b.real());
+    };
+    const int64_t nb = n / R, tw_step = n / (ns * R);
+    cd wr[R];   // W_R^q
+    for (int q = 0; q < R; ++q) wr[q] = unit_root(q, R);
+    for (int64_t g = 0; g < nb / ns; ++g) {
+        for (int64_t k = 0; k < ns; ++k) {
+            const int64_t j = g * ns + k;
+            cd v[R], o[R];
+            v[0] = src[j];
+            for (int r = 1; r < R; ++r) v[r] = k ? mul(src[j + r * nb], root(r * k * tw_step)) : src[j + r * nb];
+            if constexpr (R == 4) {
+                const cd a = v[0] + v[2], b = v[0] - v[2], c = v[1] + v[3];
+                const cd d(v[1].imag() - v[3].imag(), v[3].real() - v[1].real());   // -i (v1 - v3)
+                o[0] = a + c; o[1] = b + d; o[2] = a - c; o[3] = b - d;
+            } else if constexpr (R == 2) {
+                o[0] = v[0] + v[1];
+                o[1] = v[0] - v[1];
+            } else {
+                for (int q = 0; q < R; ++q) {
+                    cd acc = v[0];
+                    for (int r = 1; r < R; ++r) acc += mul(v[r], wr[(r * q) % R]);
+                    o[q] = acc;
+                }
+            }
+            const int64_t base = g * ns * R + k;
+            for (int q = 0; q < R; ++q) dst[base + q * ns] = o[q];
+        }
+    }
+}
+
+// Forward DFT in FP64 of a length n = 2^a 3^b 5^c 7^d, used for the
+// Bluestein kernel spectrum: iterative Stockham passes of radix 4, 2, 3, 5
+// and 7 between two buffers (natural-order output, no bit reversal), with
+// the twiddles W_n^e formed from two tables of exactly reduced roots
+// (hi[e >> s] times lo[e & (2^s - 1)], 2^s >= sqrt(n)).
 static void fft_host(std::vector<cd>& x) {
     const int64_t n = (int64_t)x.size();
     if (n <= 1) return;
-    int64_t p = 0;
-    for (int q : {4, 2, 3, 5, 7})
-        if (n % q == 0) { p = q; break; }
-    if (p == 0) {   // not smooth: direct O(n^2) (never hit by the planner)
+    std::vector<int> radices;
+    int64_t m = n;
+    while (m % 4 == 0) { radices.push_back(4); m /= 4; }
+    while (m % 2 == 0) { radices.push_back(2); m /= 2; }
+    for (int p : {3, 5, 7})
+        while (m % p == 0) { radices.push_back(p); m /= p; }
+    if (m != 1) {   // not smooth: direct O(n^2) (the planner never asks for it)
         std::vector<cd> y(n);
         for (int64_t k = 0; k < n; ++k) {
-            cd s = 0;
-            for (int64_t j = 0; j < n; ++j) s += x[j] * unit_root(j * k % n, n);
-            y[k] = s;
+            cd acc = 0;
+            for (int64_t j = 0; j < n; ++j) acc += x[j] * unit_root(j * k % n, n);
+            y[k] = acc;
         }
         x.swap(y);
         return;
     }
-    const int64_t m = n / p;
-    std::vector<std::vector<cd>> sub(p, std::vector<cd>(m));
-    for (int64_t j = 0; j < m; ++j)
-        for (int64_t r = 0; r < p; ++r) sub[r][j] = x[j * p + r];
-    for (auto& s : sub) fft_host(s);
-    for (int64_t k = 0; k < m; ++k)
-        for (int64_t q = 0; q < p; ++q) {
-            cd s = 0;
-            const int64_t kk = k + q * m;
-            for (int64_t r = 0; r < p; ++r) s += sub[r][k] * unit_root(r * kk % n, n);
-            x[kk] = s;
+    int sh = 0;
+    while ((int64_t(1) << (2 * sh)) < n) ++sh;
+    const int64_t lo_n = int64_t(1) << sh, hi_n = (n + lo_n - 1) / lo_n;
+    std::vector<cd> lo(lo_n), hi(hi_n);
+    for (int64_t e = 0; e < lo_n; ++e) lo[e] = unit_root(e, n);
+    for (int64_t e = 0; e < hi_n; ++e) hi[e] = unit_root(e * lo_n, n);
+    auto root = [&](int64_t e) {   // W_n^e, e < n
+        const cd a = hi[e >> sh], b = lo[e & (lo_n - 1)];
+        return cd(a.real() * b.real() - a.imag() * b.imag(), a.real() * b.imag() + a.imag() * b.real());
+    };
+    std::vector<cd> y(n);
+    cd* src = x.data();
+    cd* dst = y.data();
+    int64_t ns = 1;
+    for (const int R : radices) {
+        switch (R) {
+            case 4: host_stage<4>(src, dst, n, ns, root); break;
+            case 2: host_stage<2>(src, dst, n, ns, root); break;
+            case 3: host_stage<3>(src, dst, n, ns, root); break;
+            case 5: host_stage<5>(src, dst, n, ns, root); break;
+            default: host_stage<7>(src, dst, n, ns, root); break;
         }
+        std::swap(src, dst);
+        ns *= R;
+    }
+    if (src != x.data()) std::copy(src, src + n, x.data());
 }
 
 // ---------------------------------------------------------------------------

@@ -367,3 +367,16 @@ def test_checked_execute_refuses_short_buffers(vdev):
     assert ws > 0
     with pytest.raises(ValueError, match="workspace of"):
         big.native_plan.execute_checked(fake, n, fake, n, 0, fake, ws - 1, None)
+
+
+def test_large_prime_plans_quickly(vdev):
+    # the Bluestein kernel spectrum of a prime near 2^24 (M = 5 * 2^23) is an
+    # FP64 host FFT of 42 M points: iterative radix passes take ~5 s for the
+    # whole plan (a recursive one took ~50 s)
+    import time
+
+    t0 = time.perf_counter()
+    fft = FFT(ArrayMetadata((16777259,), numpy.complex64))
+    fft.compile(vdev)
+    assert time.perf_counter() - t0 < 25.0
+    assert fft.native_plan.passes[0].axis_len == 5 * 2**23
