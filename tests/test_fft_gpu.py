@@ -425,3 +425,15 @@ def test_one_plan_on_two_streams(queue):
     for k in range(2):
         err = oracle.rel_l2(outs[k].get(queue), oracle.fftn(datas[k], (1,)))
         assert err <= oracle.tolerance(shape, (1,)), (k, err)
+
+
+@pytest.mark.parametrize("n", [2**26, 5 * 2**23, 2**24 + 43],
+                         ids=["2^26", "5x2^23", "prime 2^24+43"])
+def test_single_long_transform(queue, n):
+    """One 1-D transform of 2^24 - 2^26 points (no batch dimension, so no
+    chunks): four streaming passes, a radix-5 chain, and a prime through
+    Bluestein chains over M = 5 * 2^23."""
+    data = oracle.test_array((n,), seed=n % 1000)
+    fwd, fft = run_fft(queue, data, (0,), inplace=False)
+    assert len(fft.native_plan.passes) >= 3 and fft.native_plan.chunk_rows == 0
+    assert oracle.rel_l2(fwd, oracle.fftn(data, (0,), workers=16)) <= oracle.tolerance((n,), (0,))
