@@ -25,6 +25,7 @@ import json
 import os
 import statistics
 from collections.abc import Iterable, Sequence
+from typing import Any
 from dataclasses import asdict
 from pathlib import Path
 
@@ -130,7 +131,7 @@ def autotune_batch(
 # single-transform plans
 # ---------------------------------------------------------------------------
 
-def _knob_values(native: NativeFftPlan, elements: int) -> dict[str, list]:
+def _knob_values(native: NativeFftPlan, nbytes: int) -> dict[str, list]:
     """Values worth trying for each knob of a plan, from its default pass list."""
     passes = native.passes
     by_axis: dict[int, list[int]] = {}
@@ -155,14 +156,14 @@ def _knob_values(native: NativeFftPlan, elements: int) -> dict[str, list]:
         knobs["col_tile"] = [0] + [c for c in (16, 32, 64) if c not in cols]
     if rows:
         knobs["row_tile"] = [0] + [c for c in (1, 2, 4, 8) if c not in rows]
-    if native.chunk_rows or 16 * elements > (256 << 20):
+    if native.chunk_rows or 2 * nbytes > (256 << 20):
         knobs["chunk_bytes"] = [-1, 0, 64 << 20, 128 << 20, 256 << 20]
     knobs["fixed_modes"] = [-1, 0]
     return knobs
 
 
-def _plan_key(device: Device, shape: Sequence[int], axes: Sequence[int]) -> str:
-    return f"{device.name}|{tuple(shape)}|{tuple(axes)}"
+def _plan_key(device: Device, shape: Sequence[int], axes: Sequence[int], dtype: Any) -> str:
+    return f"{device.name}|{tuple(shape)}|{tuple(axes)}|{dtype}"
 
 
 def autotune_fft(
@@ -172,38 +173,47 @@ def autotune_fft(
     reps: int = 10,
     warmup: int = 10,
     use_cache: bool = True,
+    dtype: Any = "complex64",
 ) -> tuple[PlanTuning, list[tuple[PlanTuning, float]]]:
-    """The fastest :class:`PlanTuning` for a complex64 C-contiguous array of
-    ``shape`` transformed over ``axes``, and every (tuning, ms) measured
-    (empty when the answer came from the cache).
+    """The fastest :class:`PlanTuning` for a C-contiguous complex64 or
+    complex128 array of ``shape`` transformed over ``axes``, and every
+    (tuning, ms) measured (empty when the answer came from the cache).
 
     One coordinate sweep from the planner's own plan: each knob in turn takes
     each of its values with the others at the best found so far, and a value
-    replaces the current one only when it is more than 2 % faster."""
+    replaces the current one only when it is more than 2 % faster.  A
+    candidate the planner or the run-time compiler refuses is skipped."""
     import math
 
     import numpy
 
     shape = tuple(int(n) for n in shape)
     axes = tuple(range(len(shape))) if axes is None else tuple(int(a) for a in axes)
-    key = _plan_key(queue.device, shape, axes)
+    dtype = numpy.dtype(dtype)
+    if dtype not in (numpy.complex64, numpy.complex128):
+        raise ValueError(f"plans are tuned for complex64 or complex128 arrays, not {dtype}")
+    key = _plan_key(queue.device, shape, axes, dtype)
     path = _cache_file("fft_autotune.json")
     cache = _read_cache(path) if use_cache else {}
     if key in cache:
         return PlanTuning(**cache[key]["tuning"]), []
 
-    meta = ArrayMetadata(shape, numpy.complex64)
-    elements = math.prod(shape)
+    meta = ArrayMetadata(shape, dtype)
     rng = numpy.random.default_rng(0)
-    host = (rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)).astype(numpy.complex64)
+    host = (rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)).astype(dtype)
     inp = Array.from_host(queue, host)
-    out = Array.empty(queue.device, shape, numpy.complex64)
+    out = Array.empty(queue.device, shape, dtype)
     ref_out: numpy.ndarray | None = None
-    bound = 2 * 2e-7 * max(1.0, math.log2(math.prod(shape[a] for a in axes)))
+    # twice the stated bound of the element type (two plans, each within it)
+    unit = 2e-7 if dtype == numpy.complex64 else 5e-16
+    bound = 2 * unit * max(1.0, math.log2(math.prod(shape[a] for a in axes)))
 
     def measure(t: PlanTuning) -> float | None:
         nonlocal ref_out
-        plan = NativeFftPlan(meta, meta, axes, 0, None if t == PlanTuning() else t)
+        try:
+            plan = NativeFftPlan(meta, meta, axes, 0, None if t == PlanTuning() else t)
+        except (ValueError, RuntimeError):
+            return None
         ws = Array.empty(queue.device, (plan.workspace_bytes,), numpy.uint8) if plan.workspace_bytes else None
         run = lambda: plan.execute(out.ptr, inp.ptr, 0, ws.ptr if ws is not None else None, queue.stream)  # noqa: E731
         for _ in range(warmup):
@@ -228,9 +238,11 @@ def autotune_fft(
 
     best = PlanTuning()
     best_ms = measure(best)
+    if best_ms is None:
+        raise ValueError(f"no plan for {shape} over axes {axes}")
     results: list[tuple[PlanTuning, float]] = [(best, best_ms)]
     default_plan = NativeFftPlan(meta, meta, axes, _lib.RK_FFT_DRY_RUN)
-    for knob, values in _knob_values(default_plan, elements).items():
+    for knob, values in _knob_values(default_plan, host.nbytes).items():
         for v in values:
             cand = PlanTuning(**{**asdict(best), knob: v})
             if cand == best:

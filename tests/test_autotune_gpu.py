@@ -52,3 +52,17 @@ def test_autotune_fft_picks_a_correct_plan_and_caches(queue, tmp_path, monkeypat
     # the second call answers from the cache without timing anything
     again, measured = autotune_fft(queue, shape, axes)
     assert again == best and measured == []
+
+
+def test_autotune_fft_complex128(queue, tmp_path, monkeypatch):
+    from reikna_amd.fft.autotune import autotune_fft
+
+    monkeypatch.setenv("RK_CACHE_DIR", str(tmp_path))
+    shape, axes = (16, 2**16), (1,)
+    best, results = autotune_fft(queue, shape, axes, reps=3, warmup=1, dtype="complex128")
+    assert results and best in dict(results)
+    data = oracle.test_array(shape, seed=73).astype(numpy.complex128)
+    res, plan = tuned_result(queue, data, axes, best)
+    assert plan.dtype == numpy.complex128
+    bound = 5e-16 * 16
+    assert oracle.rel_l2(res, numpy.fft.fft(data, axis=1)) <= bound
