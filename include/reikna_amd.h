@@ -259,6 +259,8 @@ typedef struct {
                              between stages stay inside it (row mapping) */
     int pipe;             /* 1 if the kernel is persistent and loads its next tile
                              while it computes the current one */
+    int occ;              /* 1 if the kernel's launch bounds ask for as many resident
+                             work-groups per CU as its LDS allows */
 } rk_fft_pass_info;
 
 int rk_fft_plan_num_passes(const rk_fft_plan* plan, int* n);

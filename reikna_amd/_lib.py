@@ -175,6 +175,7 @@ class FftPassInfo(ctypes.Structure):
         ("nt_store", ctypes.c_int),
         ("wave", ctypes.c_int),
         ("pipe", ctypes.c_int),
+        ("occ", ctypes.c_int),
     ]
 
 

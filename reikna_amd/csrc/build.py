@@ -139,7 +139,8 @@ VAR_MODE_DF = 1 << 9                 # load/store modes DIRECT/FLAT compiled in 
 VAR_MODE_FF = 1 << 10                # load/store modes FLAT/FLAT compiled in (RK_VAR_MODE_FF)
 VAR_NT_LOAD = 1 << 11                # nontemporal loads only (RK_VAR_NT_LOAD)
 VAR_NT_STORE = 1 << 12               # nontemporal stores only (RK_VAR_NT_STORE)
-FACTOR_MAX = 256                     # longest pass of a multi-pass chain (rk_fft_plan.cpp factor_max)
+VAR_OCC = 1 << 16                    # launch bounds for the LDS-allowed residency (RK_VAR_OCC)
+FACTOR_MAX = 1024                    # longest pass of a complex64 multi-pass chain (rk_fft_plan.cpp factor_max)
 
 
 def catalog_entries() -> list[tuple[int, int, int]]:
@@ -197,6 +198,14 @@ def catalog_entries() -> list[tuple[int, int, int]]:
                 ents.add((length, c, v | VAR_MODE_DD))
             else:
                 ents.add((length, c, v | VAR_MODE_DD))
+    # the 16 x 1024 column tiles of two-pass chains (work-groups of 1024
+    # threads) with DIRECT/DIRECT or DIRECT/FLAT compiled in run with launch
+    # bounds for two work-groups per CU (rk_fft_plan.cpp occ_default)
+    for length, c, v in list(ents):
+        base = v & ~(nt | VAR_MODE_DD | VAR_MODE_DF | VAR_MODE_FF)
+        if (length <= 1024 and c * threads_per_instance(length) == 1024 and base in (VAR_PLAIN, VAR_TWIDDLE)
+                and v & (VAR_MODE_DD | VAR_MODE_DF)):
+            ents.add((length, c, v | VAR_OCC))
     # single-pass row tiles of such lengths (one thread per R0 elements of an
     # instance, at least 32 of them: shorter rows are streamed flat): the
     # 1000-point tile holds 80 VGPRs with every mode and 62 with DIRECT/DIRECT

@@ -286,6 +286,16 @@ template <int L> struct Plan {
 
 // LDS layout of one instance: one padding slot every 16 complex values.
 RK_HD constexpr int lds_pad(int p) { return p + (p >> 4); }
+// lds_pad(b + c) for a run-time b >= 0 with b % 16 < BM and a c >= 0 that is a
+// constant once the caller's loops are unrolled.  When b % 16 + c % 16 < 16
+// no padding slot falls between b and b + c, and the padded position is
+// lds_pad(b) + lds_pad(c): one address register per exchange, the constant in
+// the LDS instruction's offset field, where the plain form costs a register
+// and three instructions per value (the compiler does not split a shift of a
+// sum whose low bits it cannot see).
+template <int BM> RK_HD constexpr int lds_pad_add(int b, int c) {
+    return (BM - 1) + (c & 15) < 16 ? lds_pad(b) + lds_pad(c) : lds_pad(b + c);
+}
 // Base row pitch of the instances of a tile: odd, so that lanes walking
 // adjacent instances hit distinct banks.
 RK_HD constexpr int lds_pitch_base(int L) {
@@ -735,13 +745,13 @@ __device__ __forceinline__ void lds_exchange(void* base, const X* w, X* r, FW wp
 // where stage s+1 reads them; after the last stage v[q*R + r] holds output
 // k = j + r*NB.
 template <int L, int S>
-__device__ __forceinline__ bool stage_valid(int t, int q) {
+RK_HD constexpr bool stage_valid(int t, int q) {
     using P = Plan<L>;
     return (P::nb(S) % P::T) == 0 || (t + q * P::T) < P::nb(S);
 }
 
 template <int L, int S>
-__device__ __forceinline__ int stage_in_pos(int t, int i) {   // input position of v[i]
+RK_HD constexpr int stage_in_pos(int t, int i) {   // input position of v[i]
     using P = Plan<L>;
     constexpr int R = P::radix(S);
     const int q = i / R, r = i % R;
@@ -749,7 +759,7 @@ __device__ __forceinline__ int stage_in_pos(int t, int i) {   // input position 
 }
 
 template <int L, int S>
-__device__ __forceinline__ int stage_out_pos(int t, int i) {  // output position of v[i]
+RK_HD constexpr int stage_out_pos(int t, int i) {  // output position of v[i]
     using P = Plan<L>;
     constexpr int R = P::radix(S);
     constexpr int NS = P::ns(S);
@@ -757,6 +767,61 @@ __device__ __forceinline__ int stage_out_pos(int t, int i) {  // output position
     const int j = t + q * P::T;
     return (j / NS) * NS * R + (j % NS) + r * NS;
 }
+
+// The padded LDS positions of stage_in_pos / stage_out_pos (lds_pad of them),
+// written as a per-thread part plus a per-value constant where lds_pad_add
+// allows.  Input: t + (q*T + r*NB) with t < T.  Output: when T % NS == 0,
+// j / NS = t / NS + q*T/NS and j % NS = t % NS, so the position is
+// (t/NS)*NS*R + t%NS + (q*T*R + r*NS), whose per-thread part is below NS
+// modulo 16 when NS < 16 and NS*R % 16 == 0; when the stage has no more
+// butterflies than NS (j < NS) it is t + (q*T + r*NS).
+template <int L, int S>
+RK_HD constexpr int stage_in_lds(int t, int i) {
+    using P = Plan<L>;
+    constexpr int R = P::radix(S);
+    constexpr int T = P::T;
+    const int q = i / R, r = i % R;
+    return lds_pad_add<(T < 16 ? T : 16)>(t, q * T + r * P::nb(S));
+}
+
+template <int L, int S>
+RK_HD constexpr int stage_out_lds(int t, int i) {
+    using P = Plan<L>;
+    constexpr int R = P::radix(S);
+    constexpr int NS = P::ns(S);
+    constexpr int T = P::T;
+    const int q = i / R, r = i % R;
+    if constexpr (T % NS == 0) {
+        constexpr int BM = (NS < 16 && (NS * R) % 16 == 0) ? NS : 16;
+        return lds_pad_add<BM>((t / NS) * NS * R + t % NS, q * T * R + r * NS);
+    } else if constexpr (P::nb(S) <= NS) {
+        return lds_pad_add<(T < 16 ? T : 16)>(t, q * T + r * NS);
+    } else {
+        return lds_pad(stage_out_pos<L, S>(t, i));
+    }
+}
+
+// Padded LDS position, within a tile of pitch LP, of element it of thread tid
+// on an instance-linear (FLAT) side: e = tid + it*NT is instance e / L,
+// position e % L.  When NT is a multiple of L every thread stays on one
+// position of its instances; when L is a multiple of NT, iteration it covers
+// positions (it*NT) % L + tid of instance (it*NT) / L.
+template <int L, int NT, int LP>
+RK_HD constexpr int flat_lds(int tid, int it) {
+    if constexpr (NT % L == 0) {
+        return (tid / L) * LP + lds_pad(tid % L) + it * (NT / L) * LP;
+    } else if constexpr (L % NT == 0) {
+        return ((it * NT) / L) * LP + lds_pad_add<(NT < 16 ? NT : 16)>(tid, (it * NT) % L);
+    } else {
+        const int e = tid + it * NT;
+        const int ce = e / L;
+        return ce * LP + lds_pad(e - ce * L);
+    }
+}
+
+// Padded LDS position of element it of a thread at position ps < T of an
+// instance on a staged COLUMN / ROW side: ps + it*T.
+template <int T> RK_HD constexpr int col_lds(int ps, int it) { return lds_pad_add<(T < 16 ? T : 16)>(ps, it * T); }
 
 template <int L, int S, bool SPLIT, int VM, bool WAVE = false, class X>
 __device__ __forceinline__ void run_stages(X (&v)[VM], void* my, int t, const X* twl) {
@@ -786,8 +851,8 @@ __device__ __forceinline__ void run_stages(X (&v)[VM], void* my, int t, const X*
         constexpr int Q1 = P::q(S + 1);
         lds_exchange<SPLIT, Q * R, Q1 * R1, WAVE>(
             my, v, v,
-            [&](int i) { return stage_valid<L, S>(t, i / R) ? lds_pad(stage_out_pos<L, S>(t, i)) : -1; },
-            [&](int i) { return stage_valid<L, S + 1>(t, i / R1) ? lds_pad(stage_in_pos<L, S + 1>(t, i)) : -1; });
+            [&](int i) { return stage_valid<L, S>(t, i / R) ? stage_out_lds<L, S>(t, i) : -1; },
+            [&](int i) { return stage_valid<L, S + 1>(t, i / R1) ? stage_in_lds<L, S + 1>(t, i) : -1; });
         run_stages<L, S + 1, SPLIT, VM, WAVE>(v, my, t, twl);
     }
 }
@@ -870,7 +935,7 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
 
     // ---- stage-0 inputs: element p = t + r*T of instance c ----
     const Inst ic = decompose<L>(a, inst0 + c);
-    auto own_pos = [&](int i) { return cbase + lds_pad(stage_in_pos<L, 0>(t, i)); };
+    auto own_pos = [&](int i) { return cbase + stage_in_lds<L, 0>(t, i); };
     if (PRE || load_mode == RK_MODE_DIRECT) {
         const Walk w = in_walk(a, ic, t);
         if constexpr (PRE) {
@@ -905,8 +970,7 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
         }
         lds_exchange<SPLIT, R0, R0>(
             lds, g, v,
-            [&](int it) { const int e = tid + it * NT; const int ce = e / L; return ce * LP + lds_pad(e - ce * L); },
-            own_pos);
+            [&](int it) { return flat_lds<L, NT, LP>(tid, it); }, own_pos);
     } else {
         // COLUMN (under the row mapping) or ROW (under the column mapping)
         const Inst ics = decompose<L>(a, inst0 + cs);
@@ -916,13 +980,13 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
         pre_ops<V, R0>(a, ics, w, g, [](int it) { return it * T; });
         if constexpr (TWID) apply_pass_twiddles<R0>(a, ics.l, ps, T, g);
         lds_exchange<SPLIT, R0, R0>(
-            lds, g, v, [&](int it) { return cs * LP + lds_pad(ps + it * T); }, own_pos);
+            lds, g, v, [&](int it) { return cs * LP + col_lds<T>(ps, it); }, own_pos);
     }
 
     // LDS of this thread's instance (the stages index it relative to the instance)
     void* my = SPLIT ? (void*)(reinterpret_cast<F*>(lds) + cbase) : (void*)(lds + cbase);
     auto last_out = [&](int i) {
-        return stage_valid<L, SL>(t, i / RL) ? cbase + lds_pad(stage_out_pos<L, SL>(t, i)) : -1;
+        return stage_valid<L, SL>(t, i / RL) ? cbase + stage_out_lds<L, SL>(t, i) : -1;
     };
     const X* twl = table<X>(a.twl);
     run_stages<L, 0, SPLIT, VM, WAVE>(v, my, t, twl);
@@ -959,11 +1023,7 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
         }
     } else if (store_mode == RK_MODE_FLAT) {
         // no post-ops beyond sign and scale
-        lds_exchange<SPLIT, QL * RL, R0>(lds, v, g, last_out, [&](int it) {
-            const int e = tid + it * NT;
-            const int ce = e / L;
-            return ce * LP + lds_pad(e - ce * L);
-        });
+        lds_exchange<SPLIT, QL * RL, R0>(lds, v, g, last_out, [&](int it) { return flat_lds<L, NT, LP>(tid, it); });
         const rk_i64 base = a.out_off + (rk_i64)inst0 * L;
 #pragma unroll
         for (int it = 0; it < R0; ++it) {
@@ -976,8 +1036,7 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
             }
         }
     } else {
-        lds_exchange<SPLIT, QL * RL, R0>(lds, v, g, last_out,
-                                         [&](int it) { return cs * LP + lds_pad(ps + it * T); });
+        lds_exchange<SPLIT, QL * RL, R0>(lds, v, g, last_out, [&](int it) { return cs * LP + col_lds<T>(ps, it); });
         const Inst ics = decompose<L>(a, inst0 + cs);
         if (ics.valid) {
             const Walk w = out_walk(a, ics, ps);

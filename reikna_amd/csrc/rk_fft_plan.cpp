@@ -68,15 +68,22 @@ static int64_t blue_single_max() {
 
 // Longest DFT of one pass in a multi-pass chain.  Column-style passes need
 // 16 adjacent instances per tile for 128-byte accesses, so a pass of length L
-// holds a 16 x L tile in LDS: up to L = 256 (32 KB) several work-groups share
-// a CU and every pass streams at copy speed, while L = 1024 tiles (128 KB)
-// leave one work-group per CU and run ~60% slower.  RK_FACTOR_MAX overrides.
+// holds a 16 x L tile.  Complex64 chains take passes up to 1024 points: the
+// 16 x 1024 tile (1024 threads, 68 KB of LDS through re/im halves) fits two
+// work-groups per CU in 64 VGPRs (launch bounds, occ below), and a chain
+// then needs one pass fewer from 2^17 to 2^20 points.  Measured per 512 MiB
+// transform, three passes of <= 256 against two of <= 1024: 512 x 2^17 0.445
+// against 0.347 ms, 256 x 2^18 0.449 / 0.376, 128 x 2^19 0.446 / 0.410,
+// 64 x 2^20 0.508 / 0.448, 64 x 10^6 0.639 / 0.571; from 2^21 points both
+// limits give three passes.  Complex128 tiles hold twice the registers and
+// keep 256.  RK_FACTOR_MAX overrides.
 static int64_t factor_max() {
     static const int64_t v = [] {
         const char* e = getenv("RK_FACTOR_MAX");
-        return e && *e ? (int64_t)atoll(e) : (int64_t)256;
+        return e && *e ? (int64_t)atoll(e) : (int64_t)0;
     }();
-    return v;
+    if (v) return v;
+    return t_elem_bytes == 16 ? 256 : 1024;
 }
 
 static bool length_ok(const ChainLimits& lim, int64_t n) { return lim.length_ok ? lim.length_ok(n) : is_smooth(n); }
@@ -661,7 +668,10 @@ static rk_u32 xcd_count() {
 // the neighbours meet in that XCD's L2 instead of each fetching whole lines
 // from HBM (RK_XCD_GROUP overrides the group size; 1 turns it off).  Measured
 // on 64 x 2^20 as two 1024-point passes with C = 4: 953 us without groups,
-// 715 us with groups of 4, 669 us with groups of 8.
+// 715 us with groups of 4, 669 us with groups of 8.  Tiles of 16 instances
+// go in pairs (two tiles: 256 contiguous bytes of each row): 1024 x 2^16 (two
+// 256-point passes) 0.326 against 0.310 ms, 64 x 2^20 (two 1024-point
+// passes) 0.454 against 0.448 ms.
 static void set_xcd_group(PassPlan& pp) {
     static const int forced = [] {
         const char* e = getenv("RK_XCD_GROUP");
@@ -670,7 +680,7 @@ static void set_xcd_group(PassPlan& pp) {
     rk_u32 g = 1;
     if (forced > 0)
         g = (rk_u32)forced;
-    else if ((pp.args.cmap || pp.args.load_mode == RK_MODE_COLUMN || pp.args.store_mode == RK_MODE_COLUMN) && pp.C < 16)
+    else if ((pp.args.cmap || pp.args.load_mode == RK_MODE_COLUMN || pp.args.store_mode == RK_MODE_COLUMN) && pp.C <= 16)
         g = (rk_u32)(32 / pp.C);
     if (g > 1) {
         pp.args.xcd_group = g;
@@ -825,14 +835,26 @@ static bool half_enabled() {
     return v;
 }
 
-// Passes of at least RK_OCC points run kernels whose registers are bounded
-// so that as many work-groups as their LDS allows stay resident (0: off).
+// Kernels whose launch bounds ask for as many resident work-groups per CU as
+// the tile's LDS allows.  By default: complex64 plain and twiddle passes of
+// work-groups of 1024 threads (16 waves) of at most 1024 points with
+// DIRECT/DIRECT or DIRECT/FLAT compiled in, the 16 x 1024 column tiles of
+// two-pass chains: two work-groups per CU in 64 VGPRs without spills (their
+// generic-mode kernels spill 47-55 registers under the same bound).
+// Measured on 64 x 2^20: 0.463 ms with the bound, 0.530 without.  RK_OCC=n
+// instead bounds every pass of at least n points (0: none).
 static int occ_min_length() {
     static const int v = [] {
         const char* e = getenv("RK_OCC");
-        return e && *e ? atoi(e) : 0;
+        return e && *e ? atoi(e) : -1;
     }();
     return v;
+}
+
+static bool occ_default(const PassPlan& pp) {
+    const int mbit = fixed_mode_bit(pp);
+    return pp.es == 8 && (pp.variant == VAR_PLAIN || pp.variant == VAR_TWIDDLE) && pp.L <= 1024 &&
+           (int64_t)pp.C * threads_per_instance(pp.L) == 1024 && (mbit == VAR_MODE_DD || mbit == VAR_MODE_DF);
 }
 
 static void choose_wave(PassPlan& pp, uint32_t flags) {
@@ -1125,7 +1147,7 @@ static int plan_create(const rk_fft_desc* d, rk_fft_plan** out) {
         }
         pp.pipe = pipe_enabled() && pp.args.load_mode == RK_MODE_DIRECT && pp.loops.empty();
         pp.half = half_enabled() && (int64_t)pp.C * lds_pitch_host(pp.L) * pp.es <= 80 * 1024;
-        pp.occ = occ_min_length() > 0 && pp.L >= occ_min_length();
+        pp.occ = occ_min_length() >= 0 ? occ_min_length() > 0 && pp.L >= occ_min_length() : occ_default(pp);
         rc = find_kernel(pp, d->flags);
         if (rc != RK_OK) return rc;
         set_xcd_group(pp);
@@ -1431,6 +1453,7 @@ extern "C" int rk_fft_plan_pass_info(const rk_fft_plan* plan, int pass, rk_fft_p
     info->lds_bytes = pp.lds_bytes;
     info->jit = pp.jit ? 1 : 0;
     info->variant = pp.variant;
+    info->occ = pp.occ ? 1 : 0;
     info->stream = (pp.nt_load && pp.nt_store) ? 1 : 0;
     info->nt_load = pp.nt_load ? 1 : 0;
     info->nt_store = pp.nt_store ? 1 : 0;

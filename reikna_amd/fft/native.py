@@ -43,6 +43,7 @@ class PassInfo:
     nt_store: bool = False      # nontemporal stores
     wave: bool = False          # whole instances per wave, wave-local stage exchanges
     pipe: bool = False          # persistent kernel that loads its next tile while computing one
+    occ: bool = False           # launch bounds for as many resident work-groups as the LDS allows
 
 
 @dataclass(frozen=True)
@@ -157,7 +158,7 @@ class NativeFftPlan:
                     stream=bool(info.stream),
                     fixed_modes=bool(info.fixed_modes),
                     nt_load=bool(info.nt_load), nt_store=bool(info.nt_store), wave=bool(info.wave),
-                    pipe=bool(info.pipe),
+                    pipe=bool(info.pipe), occ=bool(info.occ),
                 )
             )
         return out

@@ -33,14 +33,16 @@ def test_plan_single_pass_rows(vdev):
 
 
 def test_plan_multi_pass_large_axis(vdev):
-    ps = passes((64, 2**20), (1,), dev=vdev)
-    # fewest passes with every pass length <= 256 (16-column tiles of <= 32 KB)
-    assert len(ps) == 3 and numpy.prod([p.L for p in ps]) == 2**20 and max(p.L for p in ps) <= 256
-    assert ps[0].load_mode == "direct" and ps[0].store_mode == "flat"     # column load, contiguous store
-    for p in ps[1:]:
-        assert p.load_mode == "direct" and p.store_mode == "direct"     # column both sides
-        assert p.variant & 3 == 1                                        # inter-pass twiddle variant
-    assert [(p.src, p.dst) for p in ps] == [("in", "ws0"), ("ws0", "ws1"), ("ws1", "out")]
+    # fewest passes with every pass length <= 1024 (16-column tiles)
+    for shape, n_pass in (((64, 2**20), 2), ((16, 2**22), 3)):
+        ps = passes(shape, (1,), dev=vdev)
+        assert len(ps) == n_pass and numpy.prod([p.L for p in ps]) == shape[1] and max(p.L for p in ps) <= 1024
+        assert ps[0].load_mode == "direct" and ps[0].store_mode == "flat"     # column load, contiguous store
+        for p in ps[1:]:
+            assert p.load_mode == "direct" and p.store_mode == "direct"     # column both sides
+            assert p.variant & 3 == 1                                        # inter-pass twiddle variant
+        ws = [("in", "ws0"), ("ws0", "out")] if n_pass == 2 else [("in", "ws0"), ("ws0", "ws1"), ("ws1", "out")]
+        assert [(p.src, p.dst) for p in ps] == ws
     ps = passes((64, 2**16), (1,), dev=vdev)
     assert [p.L for p in ps] == [256, 256]
 
@@ -171,7 +173,7 @@ def test_plan_streaming_accesses(vdev):
     # chains whose data exceed the 256 MB Infinity Cache run in chunks of
     # 128 MiB of intermediate: the caller's input is loaded and the output
     # stored with the nontemporal hint, the intermediate stays cached
-    for shape, rows in (((64, 2**20), 16), ((4096, 12288), 2730)):
+    for shape, rows in (((64, 2**20), 32), ((4096, 12288), 2730), ((16, 2**22), 4)):
         n, ps = chunked(shape, (1,), vdev)
         assert n == rows, (shape, n)
         assert ps[0].nt_load and not ps[0].nt_store and ps[0].src == "in"
@@ -210,6 +212,28 @@ def test_plan_fixed_mode_kernels(vdev):
     assert [p.fixed_modes for p in passes((256, 256, 256), (0, 1, 2), dev=vdev)] == [False, True, True]
 
 
+def test_plan_two_pass_chains(vdev):
+    # complex64 chains take passes of up to 1024 points: 2^17 - 2^20 points in
+    # two passes; the 16 x 1024 column tiles (1024 threads) with their modes
+    # compiled in run with launch bounds for two work-groups per CU, all from
+    # the ahead-of-time catalogue
+    for shape, ls in (((64, 2**20), [1024, 1024]), ((256, 2**18), [512, 512]), ((512, 3 * 2**16), [512, 384]),
+                      ((8, 2**20), [1024, 1024]), ((16, 2**22), [128, 256, 128])):
+        ps = passes(shape, (1,), dev=vdev)
+        assert sorted(p.L for p in ps) == sorted(ls), shape   # (384 and 512 are equally balanced)
+        assert [p.occ for p in ps] == [p.L == 1024 for p in ps], shape
+        assert all(p.fixed_modes and not p.jit for p in ps), shape
+    # no bound where the kernel would spill (generic modes) or for other widths
+    assert not any(p.occ for p in tuned_plan((64, 2**20), (1,), vdev, fixed_modes=0).passes)
+    assert not any(p.occ for p in tuned_plan((64, 2**20), (1,), vdev, col_tile=8).passes)
+    assert not any(p.occ for p in passes((1024, 1024), (1,), dev=vdev))
+    # complex128 chains keep passes of at most 256 points
+    fft = FFT(ArrayMetadata((64, 2**20), numpy.complex128), axes=(1,))
+    fft.compile(vdev)
+    assert [p.L for p in fft.native_plan.passes] == [128, 128, 64]
+    assert not any(p.occ for p in fft.native_plan.passes)
+
+
 def test_plan_xcd_groups_for_narrow_column_tiles(vdev):
     # a 4096-point strided axis: column tiles of 4 instances (the pass is planned;
     # its XCD grouping is a launch detail checked on the GPU)
@@ -228,11 +252,15 @@ def test_other_dtypes_refused():
 
 
 def test_complex128_plans(vdev):
-    # complex128 plans: the same pass chains with twice the bytes per element,
-    # so tiles of half the instances in the same LDS, twice the workspace,
-    # chunks of half the rows, and fused transformations in double2
+    # complex128 plans: the same pass chains (passes of at most 256 points,
+    # the complex64 planner held to that limit here) with twice the bytes per
+    # element, so tiles of half the instances in the same LDS, twice the
+    # workspace, chunks of half the rows, and fused transformations in double2
+    from reikna_amd.fft import PlanTuning
+
+    fm256 = PlanTuning(factor_max=256)
     for shape, axes in (((64, 2**20), (1,)), ((4, 65521), (1,)), ((32, 64, 48), (0, 1, 2)), ((1024, 1024), (1,))):
-        p64 = FFT(ArrayMetadata(shape, numpy.complex64), axes=axes)
+        p64 = FFT(ArrayMetadata(shape, numpy.complex64), axes=axes, tuning=fm256)
         p128 = FFT(ArrayMetadata(shape, numpy.complex128), axes=axes)
         p64.compile(vdev)
         p128.compile(vdev)
@@ -243,7 +271,7 @@ def test_complex128_plans(vdev):
         assert b.chunk_rows * 2 == a.chunk_rows
     big = FFT(ArrayMetadata((4, 65521), numpy.complex128), axes=(1,))
     big.compile(vdev)
-    small = FFT(ArrayMetadata((4, 65521), numpy.complex64), axes=(1,))
+    small = FFT(ArrayMetadata((4, 65521), numpy.complex64), axes=(1,), tuning=fm256)
     small.compile(vdev)
     assert big.native_plan.workspace_bytes == 2 * small.native_plan.workspace_bytes
     # single-kernel Bluestein up to M = 2048 (longer convolutions as chains)
@@ -307,13 +335,14 @@ def tuned_plan(shape, axes, dev, **knobs):
 def test_plan_tuning_knobs(vdev):
     # rk_fft_plan_create_tuned: each knob changes the plan it names
     base = [p.L for p in tuned_plan((64, 2**20), (1,), vdev).passes]
-    assert base == [128, 128, 64]
-    assert [p.L for p in tuned_plan((64, 2**20), (1,), vdev, first_factor=64).passes] == [64, 128, 128]
-    assert [p.L for p in tuned_plan((64, 2**20), (1,), vdev, factor_max=1024).passes] == [1024, 1024]
+    assert base == [1024, 1024]
+    assert [p.L for p in tuned_plan((64, 2**20), (1,), vdev, factor_max=256).passes] == [128, 128, 64]
+    assert [p.L for p in tuned_plan((64, 2**20), (1,), vdev, factor_max=256, first_factor=64).passes] == [64, 128, 128]
     assert [p.L for p in tuned_plan((64, 2**20), (1,), vdev, factor_max=128).passes] == [128, 128, 64]
-    ps = tuned_plan((64, 2**20), (1,), vdev, col_tile=64).passes
+    ps = tuned_plan((64, 2**20), (1,), vdev, factor_max=256, col_tile=64).passes
     assert [p.C for p in ps] == [64, 64, 64]
-    assert tuned_plan((64, 2**20), (1,), vdev).chunk_rows == 16
+    assert tuned_plan((64, 2**20), (1,), vdev).chunk_rows == 32
+    assert tuned_plan((64, 2**20), (1,), vdev, factor_max=256).chunk_rows == 16
     assert tuned_plan((64, 2**20), (1,), vdev, chunk_bytes=0).chunk_rows == 0
     assert tuned_plan((64, 2**20), (1,), vdev, chunk_bytes=64 << 20).chunk_rows == 8
     assert not any(p.fixed_modes for p in tuned_plan((64, 2**20), (1,), vdev, fixed_modes=0).passes)
