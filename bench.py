@@ -183,6 +183,7 @@ def main(argv=None) -> int:
     data = oracle.test_array(shape, seed=seed, dist=dist)
     if args.dtype == "c128":
         data = data.astype(numpy.complex128)
+        desc = desc.replace("c64 ", "c128 ", 1)
     arr_in = Array.from_host(queue, data)
     arr_out = Array.empty_like(dev, arr_in)
     tuning, tuned = None, []
