@@ -216,7 +216,7 @@ def test_fused_transformations(queue):
     [
         ((4096, 997), (1,), None),              # single-kernel Bluestein (chirp pre-op + fused input)
         ((8, 65521), (1,), None),               # Bluestein as two multi-pass chains
-        ((64, 2**20), (1,), [0, 17, 40, 63]),   # the three-pass streaming chain
+        ((64, 2**20), (1,), [0, 17, 40, 63]),   # the chunked two-pass chain (16 x 1024 tiles)
         ((256, 256, 256), (0, 1, 2), None),     # three axes, the input fused into the first axis's pass
     ],
     ids=["4096x997", "8x65521", "64x2^20", "256^3"],
@@ -299,6 +299,32 @@ def test_large_streaming_passes(queue, shape, axes):
     data = oracle.test_array(shape, seed=sum(shape))
     fwd, fft = run_fft(queue, data, axes, inplace=False)
     assert any(p.nt_load or p.nt_store for p in fft.native_plan.passes)
+    assert oracle.rel_l2(fwd, oracle.fftn(data, axes)) <= oracle.tolerance(shape, axes)
+    inv, _ = run_fft(queue, data, axes, inverse=True, inplace=True)
+    assert oracle.rel_l2(inv, oracle.ifftn(data, axes)) <= oracle.tolerance(shape, axes)
+
+
+@pytest.mark.parametrize(
+    "shape,axes,lengths",
+    [
+        ((8, 2**20), (1,), [1024, 1024]),        # unchunked, both 16 x 1024 tiles under launch bounds
+        ((3, 2**19), (1,), [1024, 512]),         # one bounded pass, one 512-point pass
+        ((2**17, 4), (0,), [512, 256]),          # a strided axis: column passes with an inner dimension
+        ((2, 3 * 2**18), (1,), [1024, 768]),     # a radix-3 pass beside a bounded one
+        ((4, 10**6), (1,), [1000, 1000]),        # 1000-point passes (8-instance tiles)
+    ],
+    ids=["8x2^20", "3x2^19", "2^17x4-axis0", "2x3*2^18", "4x10^6"],
+)
+def test_two_pass_chains(queue, shape, axes, lengths):
+    """Complex64 chains take passes of up to 1024 points; the 16 x 1024
+    column tiles with compiled-in modes run with launch bounds for two
+    work-groups per CU.  Forward out of place and inverse in place against
+    the oracle."""
+    data = oracle.test_array(shape, seed=sum(shape) + 7)
+    fwd, fft = run_fft(queue, data, axes, inplace=False)
+    ps = fft.native_plan.passes
+    assert sorted(p.L for p in ps) == sorted(lengths)
+    assert [p.occ for p in ps] == [p.L == 1024 and p.C == 16 for p in ps]
     assert oracle.rel_l2(fwd, oracle.fftn(data, axes)) <= oracle.tolerance(shape, axes)
     inv, _ = run_fft(queue, data, axes, inverse=True, inplace=True)
     assert oracle.rel_l2(inv, oracle.ifftn(data, axes)) <= oracle.tolerance(shape, axes)
