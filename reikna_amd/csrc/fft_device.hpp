@@ -425,9 +425,11 @@ template <class X> __device__ __forceinline__ real_t<X> out_scale_of(const rk_pa
 // store(a, o, k, i, addr, v): o / i are the flat indices over the dims before /
 // after the transformed axis, m / k the axis position, addr the plain strided
 // element address.  Fused transformations (generated at run time) use o, m, i.
-// kBranchFree: loads of elements outside the data may be redirected to element
-// 0 of the buffer and discarded, so every load of a thread is issued without a
-// branch around it (a branch per load serialises them on their completion).
+// kBranchFree: a lane with nothing to load loads a valid element instead (its
+// instance's first position, or the tile's first instance: valid indices o,
+// m, i and a valid address) and discards it, so every load of a thread is
+// issued without a branch around it (a branch per load serialises them on
+// their completion).
 // Streaming variant (NT = true): every load and store carries the
 // nontemporal hint, so a pass whose input and output far exceed the 256 MB
 // Infinity Cache does not fill it and the L2 with lines nothing reads again
@@ -599,8 +601,9 @@ __device__ __forceinline__ typename IO::value_type load_at(const rk_pass_args& a
             // (a valid address, different per instance) and discards it: one
             // shared clamp address would be a hot spot for loads that bypass
             // the caches
-            const rk_i64 spare = ic.in_base + (rk_i64)(ic.jpass < a.n_valid_in ? ic.jpass : 0) * a.in_sa;
-            const X y = io.load(a, ic.o, m, ic.i, ok ? addr : spare);
+            const rk_u64 spare_m = ic.jpass < a.n_valid_in ? ic.jpass : 0;
+            const rk_i64 spare = ic.in_base + (rk_i64)spare_m * a.in_sa;
+            const X y = io.load(a, ic.o, ok ? m : spare_m, ic.i, ok ? addr : spare);
             return c_select(ok, y, zero);
         } else {
             return ok ? io.load(a, ic.o, m, ic.i, addr) : zero;
@@ -957,7 +960,8 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
             X x = cx<X>((F)0, (F)0);
             const bool ok = inst0 + ce < a.n_inst;
             if constexpr (IO::kBranchFree) {
-                const X y = io.load(a, inst0 + ce, p, 0, ok ? base + e : base);
+                // a lane past the last instance loads the tile's first element
+                const X y = io.load(a, ok ? inst0 + ce : inst0, ok ? p : 0, 0, ok ? base + e : base);
                 x = c_select(ok, y, x);
             } else {
                 if (ok) x = io.load(a, inst0 + ce, p, 0, base + e);

@@ -213,7 +213,7 @@ class _FusedFft:
             parts.append(
                 "struct RkFusedIO {\n"
                 f"    using value_type = {xtype};\n"
-                "    static constexpr bool kBranchFree = false;\n"
+                "    static constexpr bool kBranchFree = true;\n"
 
                 "    const RkLoadArgs& la;\n"
                 "    const RkStoreArgs& sa;\n"
@@ -226,8 +226,14 @@ class _FusedFft:
             # the pass's load/store modes compiled in (fft_device.hpp RK_MODES)
             modes = (f"RK_MODES({_MODE_CODES[info.load_mode]}, {_MODE_CODES[info.store_mode]})"
                      if _fixed_modes() else "0")
+            # launch bounds of the plan's own kernel for the residency the
+            # tile's LDS allows (fft_device.hpp RK_VAR_OCC, RK_VAR_F64)
+            bounds = f"{info.threads}"
+            if info.occ:
+                v = info.variant | 65536 | (131072 if xtype == "double2" else 0)
+                bounds += f", (rkfft::pass_min_waves<{info.L}, {info.C}, {v}>())"
             parts.append(
-                f'extern "C" __global__ void __launch_bounds__({info.threads}) '
+                f'extern "C" __global__ void __launch_bounds__({bounds}) '
                 "rk_fused_pass(rk_pass_args a, rk_user_blob lb, rk_user_blob sb) {\n"
                 "    const RkLoadArgs& la = *reinterpret_cast<const RkLoadArgs*>(lb.b);\n"
                 "    const RkStoreArgs& sa = *reinterpret_cast<const RkStoreArgs*>(sb.b);\n"
