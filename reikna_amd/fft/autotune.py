@@ -140,7 +140,8 @@ def _knob_values(native: NativeFftPlan, nbytes: int) -> dict[str, list]:
     multi = {a: ls for a, ls in by_axis.items() if len(ls) > 1}
     knobs: dict[str, list] = {}
     if multi:
-        knobs["factor_max"] = [0, 128, 512]
+        # the planner's limit is 1024 points for complex64 chains, 256 for complex128
+        knobs["factor_max"] = [0, 128, 256, 512, 1024]
         firsts = set()
         for ls in multi.values():
             n = 1
