@@ -159,6 +159,40 @@ def test_bench_configs(queue, shape, axes, dist):
     assert oracle.rel_l2(inv, oracle.ifftn(data, axes)) <= oracle.tolerance(shape, axes)
 
 
+def test_headline_closed_forms(queue):
+    """The 64 x 2^20 transform (two chunked 1024-point passes) against what
+    the mathematics fixes, without numpy.fft: an impulse at position p gives
+    exp(-2 pi i p m / N) exactly (rows in both 32-row chunks), Parseval's
+    identity, linearity, and the forward/inverse round trip."""
+    shape, n = (64, 2**20), 2**20
+    tol = oracle.tolerance((n,), (0,))
+    rows = (0, 17, 31, 32, 50, 63)
+    pos = {r: (r * 7919 * 13 + 5) % n for r in rows}
+    imp = numpy.zeros(shape, numpy.complex64)
+    for r, p in pos.items():
+        imp[r, p] = 1.0
+    fwd, fft = run_fft(queue, imp, (1,), inplace=False)
+    assert [p.L for p in fft.native_plan.passes] == [1024, 1024] and fft.native_plan.chunk_rows == 32
+    m = numpy.arange(n, dtype=numpy.float64)
+    for r, p in pos.items():
+        want = numpy.exp(-2j * numpy.pi * ((p * m) % n) / n)   # exact integer phase p*m mod N
+        assert oracle.rel_l2(fwd[r], want) <= tol, (r, p)
+    assert not fwd[1].any() and not fwd[62].any()   # rows without an impulse stay zero
+    x = oracle.test_array(shape, seed=97)
+    y = oracle.test_array(shape, seed=98)
+    fx, _ = run_fft(queue, x, (1,), inplace=False)
+    fy, _ = run_fft(queue, y, (1,), inplace=False)
+    a, b = numpy.complex64(0.75 - 0.5j), numpy.complex64(-1.25 + 2.0j)
+    fxy, _ = run_fft(queue, (a * x + b * y).astype(numpy.complex64), (1,), inplace=False)
+    lin = a.astype(numpy.complex128) * fx.astype(numpy.complex128) + b.astype(numpy.complex128) * fy
+    assert oracle.rel_l2(fxy, lin) <= 2 * tol
+    ex = numpy.sum(numpy.abs(x.astype(numpy.complex128)) ** 2, axis=1) * n
+    efx = numpy.sum(numpy.abs(fx.astype(numpy.complex128)) ** 2, axis=1)
+    assert numpy.max(numpy.abs(efx / ex - 1.0)) <= 2 * tol
+    back, _ = run_fft(queue, fx, (1,), inverse=True, inplace=True)
+    assert oracle.rel_l2(back, x) <= 2 * tol
+
+
 @pytest.mark.parametrize("n", [997, 1009, 4093, 4099, 8191, 12289, 65521, 131071])
 def test_bluestein_primes(queue, n):
     """Primes: single-kernel Bluestein up to M = 8192, two chains of passes above."""
