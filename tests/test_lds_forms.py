@@ -8,7 +8,9 @@ them.  This test builds a host program from the header and compares every
 form with `lds_pad` of the plain position (`stage_in_pos`, `stage_out_pos`,
 the instance-linear e = tid + it * NT and the column position ps + it * T),
 for every thread, value and stage of every 7-smooth length up to 16384 and
-tile widths 1-64 at both element sizes and both exchange layouts.  The layout
+every tile width of the kernel catalogue and the batch kernel (1-256
+instances, up to 1024 threads) at both element sizes and both exchange
+layouts.  The layout
 itself (where reikna pads its local buffers, reikna/fft/fft.py:149-213) is
 unchanged; only its arithmetic form is checked here.
 """
@@ -82,8 +84,11 @@ template <int L, int C> void check_width() {
 template <int L> void check_length() {
     check_stages<L>();
     check_width<L, 1>(); check_width<L, 2>(); check_width<L, 3>(); check_width<L, 4>();
-    check_width<L, 5>(); check_width<L, 8>(); check_width<L, 12>(); check_width<L, 16>();
-    check_width<L, 32>(); check_width<L, 64>();
+    check_width<L, 5>(); check_width<L, 6>(); check_width<L, 8>(); check_width<L, 10>();
+    check_width<L, 12>(); check_width<L, 16>(); check_width<L, 21>(); check_width<L, 25>();
+    check_width<L, 32>(); check_width<L, 42>(); check_width<L, 48>(); check_width<L, 51>();
+    check_width<L, 64>(); check_width<L, 80>(); check_width<L, 85>(); check_width<L, 128>();
+    check_width<L, 256>();
 }
 
 int main() {
@@ -116,7 +121,7 @@ def test_split_lds_forms_equal_plain_positions():
         (tmp / "check.hip").write_text(src)
         exe = tmp / "check"
         build = subprocess.run(
-            [HIPCC, "--offload-arch=gfx950", "--offload-host-only", "-O1", "-std=c++17",
+            [HIPCC, "--offload-arch=gfx950", "--offload-host-only", "-O0", "-std=c++17",
              f"-I{CSRC}", f"-I{ROOT / 'include'}", str(tmp / "check.hip"), "-o", str(exe)],
             capture_output=True, text=True, timeout=600)
         assert build.returncode == 0, build.stderr[-4000:]
