@@ -66,3 +66,28 @@ def test_autotune_fft_complex128(queue, tmp_path, monkeypatch):
     assert plan.dtype == numpy.complex128
     bound = 5e-16 * 16
     assert oracle.rel_l2(res, numpy.fft.fft(data, axis=1)) <= bound
+
+
+@pytest.mark.parametrize(
+    "shape,dtype,factor_max,lengths",
+    [
+        ((8, 2**20), numpy.complex128, 1024, [1024, 1024]),   # complex128 passes beyond the default 256
+        ((4, 2**22), numpy.complex64, 2048, [2048, 2048]),    # 2048-point column tiles (8 instances)
+        ((2, 2**22), numpy.complex128, 2048, [2048, 2048]),   # the same in FP64 (4 instances)
+    ],
+    ids=["c128-1024", "c64-2048", "c128-2048"],
+)
+def test_tuned_long_chain_passes(queue, shape, dtype, factor_max, lengths):
+    """Chain passes longer than the planner's own limit, as a tuned plan may
+    choose them (kernels from the run-time compiler): forward and inverse
+    against numpy.fft in FP64 at the element type's bound."""
+    data = oracle.test_array(shape, seed=81).astype(dtype)
+    res, plan = tuned_result(queue, data, (1,), PlanTuning(factor_max=factor_max))
+    assert [p.L for p in plan.passes] == lengths
+    n = shape[1]
+    bound = (5e-16 if dtype == numpy.complex128 else 2e-7) * numpy.log2(n)
+    assert oracle.rel_l2(res, numpy.fft.fft(data.astype(numpy.complex128), axis=1)) <= bound
+    arr = Array.from_host(queue, res.astype(dtype))
+    fft = FFT(arr, axes=(1,), tuning=PlanTuning(factor_max=factor_max))
+    fft.compile(queue.device)(queue, arr, arr, 1)
+    assert oracle.rel_l2(arr.get(queue), data) <= 2 * bound
