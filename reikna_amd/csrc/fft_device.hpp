@@ -900,9 +900,11 @@ template <int L, int C, bool HALF = false, int ES = 8> __host__ __device__ const
 // same values to the same addresses, and take part in every barrier.
 // With PRE, the tile's DIRECT loads were issued earlier (raw values in
 // `pre`, see direct_raw_loads): the tile starts from them instead of loading.
-template <int L, int C, int V, class IO, int MODES = 0, bool WAVE = false, bool PRE = false, bool HALF = false>
+template <int L, int C, int V, class IO, int MODES = 0, bool WAVE = false, bool PRE = false, bool HALF = false,
+          bool TWLDS = false>
 __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& io, typename IO::value_type* lds,
-                                              rk_u64 tile, int tid, const typename IO::value_type* pre = nullptr) {
+                                              rk_u64 tile, int tid, const typename IO::value_type* pre = nullptr,
+                                              const typename IO::value_type* twl_lds = nullptr) {
     using X = typename IO::value_type;
     using F = real_t<X>;
     constexpr int ES = (int)sizeof(X);
@@ -992,7 +994,9 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
     auto last_out = [&](int i) {
         return stage_valid<L, SL>(t, i / RL) ? cbase + stage_out_lds<L, SL>(t, i) : -1;
     };
-    const X* twl = table<X>(a.twl);
+    const X* twl;
+    if constexpr (TWLDS) twl = twl_lds;
+    else twl = table<X>(a.twl);
     run_stages<L, 0, SPLIT, VM, WAVE>(v, my, t, twl);
 
     if constexpr (BLUE) {
@@ -1053,6 +1057,14 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
 // Threads per work-group of fft_pass_body<L, C, ...>.
 template <int L, int C> __host__ __device__ constexpr int pass_threads() { return C * Plan<L>::T; }
 
+// Tiles whose stage twiddles are read from an LDS copy of the W_L table
+// instead of global memory: one thread per table entry (C * T == L), a first
+// exchange with a work-group barrier (no wave-local stages), no Bluestein
+// second transform, complex64.
+template <int L, int C, int V, bool WAVE, int ES> __host__ __device__ constexpr bool twl_in_lds() {
+    return L == 1024 && C * Plan<L>::T == L && !WAVE && !Variant<V>::BLUE && ES == 8;
+}
+
 // One work-group of a pass launch: the tile this work-group owns, with the
 // work-group's own LDS.
 template <int L, int C, int V, class IO, int MODES = 0, bool WAVE = false, bool HALF = false>
@@ -1063,7 +1075,15 @@ __device__ __forceinline__ void fft_pass_body(const rk_pass_args& a, const IO& i
     __shared__ X lds[tile_split<L, C, HALF, ES>() ? (C * LP + 1) / 2 : C * LP];
     const rk_u64 tile = xcd_tile(a) + a.tile_base;
     if (tile >= a.n_tiles) return;
-    fft_pass_tile<L, C, V, IO, MODES, WAVE, false, HALF>(a, io, lds, tile, (int)threadIdx.x);
+    if constexpr (twl_in_lds<L, C, V, WAVE, ES>()) {
+        // the stage twiddles W_L^e from LDS: one entry per thread, published
+        // by the tile's first exchange barrier (before stage 1 reads them)
+        __shared__ X twl_s[L];
+        twl_s[threadIdx.x] = table<X>(a.twl)[threadIdx.x];
+        fft_pass_tile<L, C, V, IO, MODES, WAVE, false, HALF, true>(a, io, lds, tile, (int)threadIdx.x, nullptr, twl_s);
+    } else {
+        fft_pass_tile<L, C, V, IO, MODES, WAVE, false, HALF>(a, io, lds, tile, (int)threadIdx.x);
+    }
 }
 
 // The raw DIRECT loads of one tile (what fft_pass_tile<..., PRE> starts from).
