@@ -1058,11 +1058,15 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
 template <int L, int C> __host__ __device__ constexpr int pass_threads() { return C * Plan<L>::T; }
 
 // Tiles whose stage twiddles are read from an LDS copy of the W_L table
-// instead of global memory: one thread per table entry (C * T == L), a first
-// exchange with a work-group barrier (no wave-local stages), no Bluestein
-// second transform, complex64.
-template <int L, int C, int V, bool WAVE, int ES> __host__ __device__ constexpr bool twl_in_lds() {
-    return L == 1024 && C * Plan<L>::T == L && !WAVE && !Variant<V>::BLUE && ES == 8;
+// instead of global memory (the table loads otherwise take a large share of
+// the vector-memory instructions): complex64 tiles with at least one stage
+// after the first, exchanges with work-group barriers (no wave-local stages)
+// and no Bluestein second transform, whose LDS-limited residency per CU the
+// extra L entries leave unchanged.
+template <int L, int C, int V, bool WAVE, bool HALF, int ES> __host__ __device__ constexpr bool twl_in_lds() {
+    if (WAVE || Variant<V>::BLUE || ES != 8 || Plan<L>::NSTAGE < 2) return false;
+    constexpr int base = tile_lds_bytes<L, C, HALF, ES>();
+    return (160 * 1024) / (base + L * ES) == (160 * 1024) / base;
 }
 
 // One work-group of a pass launch: the tile this work-group owns, with the
@@ -1075,11 +1079,11 @@ __device__ __forceinline__ void fft_pass_body(const rk_pass_args& a, const IO& i
     __shared__ X lds[tile_split<L, C, HALF, ES>() ? (C * LP + 1) / 2 : C * LP];
     const rk_u64 tile = xcd_tile(a) + a.tile_base;
     if (tile >= a.n_tiles) return;
-    if constexpr (twl_in_lds<L, C, V, WAVE, ES>()) {
-        // the stage twiddles W_L^e from LDS: one entry per thread, published
-        // by the tile's first exchange barrier (before stage 1 reads them)
+    if constexpr (twl_in_lds<L, C, V, WAVE, HALF, ES>()) {
+        // the stage twiddles W_L^e from LDS, published by the tile's first
+        // exchange barrier (before stage 1 reads them)
         __shared__ X twl_s[L];
-        twl_s[threadIdx.x] = table<X>(a.twl)[threadIdx.x];
+        for (int e = (int)threadIdx.x; e < L; e += pass_threads<L, C>()) twl_s[e] = table<X>(a.twl)[e];
         fft_pass_tile<L, C, V, IO, MODES, WAVE, false, HALF, true>(a, io, lds, tile, (int)threadIdx.x, nullptr, twl_s);
     } else {
         fft_pass_tile<L, C, V, IO, MODES, WAVE, false, HALF>(a, io, lds, tile, (int)threadIdx.x);
