@@ -1061,12 +1061,17 @@ template <int L, int C> __host__ __device__ constexpr int pass_threads() { retur
 // instead of global memory (the table loads otherwise take a large share of
 // the vector-memory instructions): complex64 tiles with at least one stage
 // after the first, exchanges with work-group barriers (no wave-local stages)
-// and no Bluestein second transform, whose LDS-limited residency per CU the
-// extra L entries leave unchanged.
+// and no Bluestein second transform, where the extra L entries leave the
+// tile's LDS-limited residency per CU unchanged, or cost at most 16 KB and
+// leave two work-groups per CU.  Measured on row tiles, whose residency the
+// table does lower: 4096 x 1000 20.0 -> 17.6 us, 16384 x 2048 117 -> 102 us,
+// 16384 x 1024 50.5 -> 47.7 us; 4096-point rows (a 32 KB table, half the
+// residency) 64 -> 70 us, so they keep the global table.
 template <int L, int C, int V, bool WAVE, bool HALF, int ES> __host__ __device__ constexpr bool twl_in_lds() {
     if (WAVE || Variant<V>::BLUE || ES != 8 || Plan<L>::NSTAGE < 2) return false;
     constexpr int base = tile_lds_bytes<L, C, HALF, ES>();
-    return (160 * 1024) / (base + L * ES) == (160 * 1024) / base;
+    constexpr int with = base + L * ES;
+    return (160 * 1024) / with == (160 * 1024) / base || (L * ES <= 16 * 1024 && (160 * 1024) / with >= 2);
 }
 
 // One work-group of a pass launch: the tile this work-group owns, with the
