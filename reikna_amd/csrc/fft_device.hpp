@@ -1015,7 +1015,8 @@ __device__ __forceinline__ void fft_pass_tile(const rk_pass_args& a, const IO& i
         // in registers across both transforms (the 2048-point tile: 162 -> 124
         // VGPRs, four waves per SIMD instead of two)
         asm volatile("" : "+v"(t));
-        run_stages<L, 0, SPLIT, VM, WAVE>(v, my, t, table<X>(a.twl));
+        if constexpr (TWLDS) run_stages<L, 0, SPLIT, VM, WAVE>(v, my, t, twl_lds);
+        else run_stages<L, 0, SPLIT, VM, WAVE>(v, my, t, table<X>(a.twl));
     }
 
     // ---- store: output element k = stage_out_pos of the last stage ----
@@ -1060,15 +1061,16 @@ template <int L, int C> __host__ __device__ constexpr int pass_threads() { retur
 // Tiles whose stage twiddles are read from an LDS copy of the W_L table
 // instead of global memory (the table loads otherwise take a large share of
 // the vector-memory instructions): complex64 tiles with at least one stage
-// after the first, exchanges with work-group barriers (no wave-local stages)
-// and no Bluestein second transform, where the extra L entries leave the
+// after the first and exchanges with work-group barriers (no wave-local
+// stages), where the extra L entries leave the
 // tile's LDS-limited residency per CU unchanged, or cost at most 16 KB and
 // leave two work-groups per CU.  Measured on row tiles, whose residency the
 // table does lower: 4096 x 1000 20.0 -> 17.6 us, 16384 x 2048 117 -> 102 us,
-// 16384 x 1024 50.5 -> 47.7 us; 4096-point rows (a 32 KB table, half the
-// residency) 64 -> 70 us, so they keep the global table.
+// 16384 x 1024 50.5 -> 47.7 us, the single-kernel Bluestein 4096 x 997 (both
+// of its transforms read the table) 53 -> 44 us; 4096-point rows (a 32 KB
+// table, half the residency) 64 -> 70 us, so they keep the global table.
 template <int L, int C, int V, bool WAVE, bool HALF, int ES> __host__ __device__ constexpr bool twl_in_lds() {
-    if (WAVE || Variant<V>::BLUE || ES != 8 || Plan<L>::NSTAGE < 2) return false;
+    if (WAVE || ES != 8 || Plan<L>::NSTAGE < 2) return false;
     constexpr int base = tile_lds_bytes<L, C, HALF, ES>();
     constexpr int with = base + L * ES;
     return (160 * 1024) / with == (160 * 1024) / base || (L * ES <= 16 * 1024 && (160 * 1024) / with >= 2);
