@@ -68,7 +68,7 @@ MEASURED_TRAFFIC = {
     "c1": (3.407e7, "profiles/fft_pass_c1_1024x1024.md"),
     "c2a": (5.372e8, "profiles/fft_pass_c2a_256cubed.md"),
     "c3-1000": (6.581e7, "profiles/fft_pass_c3_4096x1000.md"),
-    "c3-997": (6.627e7, "profiles/fft_pass_c3_4096x997_bluestein.md"),
+    "c3-997": (6.629e7, "profiles/fft_pass_c3_4096x997_bluestein.md"),
     "c3-12288": (1.612e9, "profiles/fft_pass_c3_4096x12288.md"),
     "c4": (4.134e8, "profiles/fft_batch_c4_512_uneven.md"),
 }
