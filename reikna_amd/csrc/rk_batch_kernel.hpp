@@ -51,12 +51,31 @@ constexpr int kLdsBytes = max_of(kCaseLds, kNumCases);
 static_assert(max_of(kCaseThreads, kNumCases) <= kThreads, "a batch tile case needs more threads than a work-group has");
 static_assert(kLdsBytes <= 64 * 1024, "batch tile cases must leave room for two work-groups per CU");
 
+// A case whose tile and W_L table fit kBatchLds reads its stage twiddles from
+// an LDS copy of the table behind its tile, as the one-pass kernels' tiles do
+// (fft_device.hpp twl_in_lds).  40 KB keeps the LDS-limited residency of the
+// largest tile alone (four work-groups per CU) and takes the chain passes of
+// up to 512 points.
+constexpr int kBatchLds = kLdsBytes > 40 * 1024 ? kLdsBytes : 40 * 1024;
+template <int L, int C> constexpr int twl_offset() { return (rkfft::tile_lds_bytes<L, C>() + 15) / 16 * 16; }
+template <int L, int C> constexpr bool batch_twl() {
+    return rkfft::Plan<L>::NSTAGE >= 2 && twl_offset<L, C>() + L * (int)sizeof(float2) <= kBatchLds;
+}
+
 __device__ __forceinline__ void run_case(rk_u32 kc, const rk_pass_args& a, float2* lds, rk_u64 tile, int tid) {
     switch (kc) {
 #define RK_CASE_RUN(id, L, C, V)                                                       \
     case id: {                                                                         \
         constexpr int nt = rkfft::pass_threads<L, C>();                                \
-        rkfft::fft_pass_tile<L, C, V>(a, rkfft::CoherentIO{}, lds, tile, tid < nt ? tid : nt - 1); \
+        if constexpr (batch_twl<L, C>()) {                                             \
+            /* published by the tile's first exchange barrier */                        \
+            float2* tw = reinterpret_cast<float2*>(reinterpret_cast<char*>(lds) + twl_offset<L, C>()); \
+            for (int e = tid; e < L; e += kThreads) tw[e] = rkfft::table<float2>(a.twl)[e]; \
+            rkfft::fft_pass_tile<L, C, V, rkfft::CoherentIO, 0, false, false, false, true>( \
+                a, rkfft::CoherentIO{}, lds, tile, tid < nt ? tid : nt - 1, nullptr, tw); \
+        } else {                                                                       \
+            rkfft::fft_pass_tile<L, C, V>(a, rkfft::CoherentIO{}, lds, tile, tid < nt ? tid : nt - 1); \
+        }                                                                              \
     } break;
         RK_BATCH_CASES(RK_CASE_RUN)
 #undef RK_CASE_RUN
@@ -110,7 +129,7 @@ __device__ __forceinline__ void clear_counters(rk_u32* c, rk_u32 groups) {
 }  // namespace
 
 extern "C" __global__ void __launch_bounds__(kThreads, RK_BATCH_MIN_WAVES) RK_BATCH_KERNEL(rk_batch_work w) {
-    __shared__ float2 lds[(kLdsBytes + 7) / 8];
+    __shared__ float2 lds[(kBatchLds + 7) / 8];
     const int tid = threadIdx.x;
     // queue position = work-group index: work-groups are dispatched in index
     // order, so every earlier position is running or finished
@@ -194,7 +213,7 @@ const std::vector<BatchCase>& cases() {
     return all;
 }
 
-int lds_bytes() { return kLdsBytes; }
+int lds_bytes() { return kBatchLds; }
 
 }  // namespace RK_BATCH_HOST
 }  // namespace rk
